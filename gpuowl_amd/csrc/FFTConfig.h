@@ -1,0 +1,68 @@
+// FFT shape and per-exponent FFT selection.
+// Mirrors the reference interface src/FFTConfig.h:24-87 (FFTShape, FFTConfig,
+// bestFit, maxBpw, carry32BPW, needsLargeCarry, isFavoredShape) so that the
+// FFT length chosen for an exponent is the reference's.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+// Format n with a K or M suffix when a multiple of 1024 (src/FFTConfig.cpp:266-282).
+std::string numberK(u32 n);
+
+class FFTShape {
+public:
+  static constexpr float MIN_BPW = 3;
+  static constexpr int NUM_BPW = 6;  // variants 000, 101, 202, 010, 111, 212
+
+  // All WIDTH x MIDDLE x HEIGHT shapes with size in [from, to], in the reference's
+  // preference order (src/FFTConfig.cpp:77-100).
+  static std::vector<FFTShape> allShapes(u32 from = 0, u32 to = u32(-1));
+  // "W:M:H", a size "6.5M", a range "6M-7M", or a comma list of these (src/FFTConfig.cpp:41-75).
+  static std::vector<FFTShape> multiSpec(const std::string& spec);
+
+  u32 width = 0, middle = 0, height = 0;
+  std::array<double, NUM_BPW> bpw{};
+
+  FFTShape(u32 w = 1, u32 m = 1, u32 h = 1);
+  explicit FFTShape(const std::string& spec);
+
+  u32 size() const { return width * middle * height * 2; }
+  u32 nW() const { return (width == 1024 || width == 256) ? 4 : 8; }
+  u32 nH() const { return (height == 1024 || height == 256) ? 4 : 8; }
+  double maxBpw() const;
+  std::string spec() const { return numberK(width) + ':' + numberK(middle) + ':' + numberK(height); }
+
+  double carry32BPW() const;
+  bool needsLargeCarry(u32 E) const { return E / double(size()) > carry32BPW(); }
+  bool isFavoredShape() const;
+};
+
+enum CarryKind : u32 { CARRY_32 = 0, CARRY_64 = 1, CARRY_AUTO = 2 };
+
+// A variant is three digits v_W v_M v_H with v_W, v_H < 3 and v_M < 2.
+constexpr u32 LAST_VARIANT = 212;
+
+struct FFTConfig {
+  // spec empty: the first shape (variants 101 then 202) whose maxExp covers E
+  // (src/FFTConfig.cpp:235-263, without a tune file). Otherwise the spec's FFT.
+  static FFTConfig bestFit(u32 E, const std::string& spec, double overdrive = 1.0);
+
+  FFTShape shape{};
+  u32 variant = LAST_VARIANT;
+  u32 carry = CARRY_AUTO;
+
+  FFTConfig() = default;
+  explicit FFTConfig(const std::string& spec);
+  FFTConfig(FFTShape shape, u32 variant, u32 carry);
+
+  std::string spec() const;
+  u32 size() const { return shape.size(); }
+  u32 maxExp() const { return u32(maxBpw() * shape.size()); }
+  double maxBpw() const;
+};

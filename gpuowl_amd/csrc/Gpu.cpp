@@ -1,0 +1,538 @@
+// Engine for one exponent; see Gpu.h. Host-side behaviour follows src/Gpu.cpp.
+#include "Gpu.h"
+
+#include "Trig.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+u32 pow2AtLeast(u32 x, u32 lo, u32 hi) {
+  u32 t = lo;
+  while (t < x && t < hi) { t *= 2; }
+  return t;
+}
+
+double2 d2(std::pair<double, double> p) { return double2{p.first, p.second}; }
+
+}  // namespace
+
+Kernel::Kernel(const std::string& name) : name_{name} { owlCheck(owl_hip_kernel_get(name.c_str(), &k_), "kernel lookup"); }
+
+void Kernel::operator()(owl_queue q, u32 gridX, u32 gridY, u32 block, u32 lds, std::initializer_list<void*> args) const {
+  std::vector<void*> a(args);
+  int rc = owl_hip_launch(q, k_, gridX, gridY, block, lds, a.data());
+  if (rc) { throw GpuError("launch " + name_ + ": " + owl_last_error()); }
+}
+
+template<typename T> T* Gpu::alloc(size_t n) {
+  owned_.push_back(std::make_unique<Buffer<char>>(std::max<size_t>(n * sizeof(T), 4)));
+  return reinterpret_cast<T*>(owned_.back()->get());
+}
+
+template<typename T> T* Gpu::upload(const std::vector<T>& v) {
+  T* p = alloc<T>(v.size());
+  owlCheck(owl_hip_write(queue_, p, v.data(), v.size() * sizeof(T), 1), "upload");
+  return p;
+}
+
+bool equals9(const Words& w) {
+  if (w.empty() || w[0] != 9) { return false; }
+  return std::all_of(w.begin() + 1, w.end(), [](u32 x) { return x == 0; });
+}
+
+Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn} {
+  owlCheck(owl_hip_set_device(opt.device), "set device");
+  W = fft.shape.width;
+  H = fft.shape.middle * fft.shape.height;
+  N = fft.size();
+  ND = N / 2;
+  if (W * H != ND) { throw std::invalid_argument("bad FFT geometry"); }
+  if (H > 4096 || W > 4096 || H < 64) { throw std::invalid_argument("FFT shape " + fft.spec() + " not supported"); }
+  double bpw = double(E) / N;
+  if (bpw < FFTShape::MIN_BPW) { throw std::invalid_argument("FFT size too large for exponent"); }
+  if (E % 32 == 0 || E < 64) { throw std::invalid_argument("exponent not supported"); }
+  useLongCarry = opt.longCarry;
+
+  // Line block R: as many lines per row-pass group as LDS allows, keeping >= 256 groups,
+  // then longer if needed for the fused carry to be stable: a block's carry-in must die
+  // within its 2R-word chain, (2R - 1) * floor(E/N) >= log2(N)/2 + 4. Below that the
+  // iteration runs with a full carry (lead-out + lead-in) every time, as the reference
+  // does below 12 bits per word (src/Gpu.cpp:522).
+  const u32 rMax = std::min<u32>(4096 / W, 16);
+  auto fusedStable = [&](u32 r) { return (2.0 * r - 1) * (E / N) >= 0.5 * std::log2(double(N)) + 4; };
+  R = opt.lineBlock;
+  if (!R) {
+    R = rMax;
+    while (R > 1 && H / R < 256) { R /= 2; }
+    while (R < rMax && !fusedStable(R) && H % (2 * R) == 0) { R *= 2; }
+  }
+  if (!R || H % R || R * W > 4096) { throw std::invalid_argument("bad line block"); }
+  nb = H / R;
+  useLongCarry = useLongCarry || !fusedStable(R);
+
+  rowPlan = fftPlan(W);
+  colPlan = fftPlan(H);
+  if (rowPlan.size() > OWL_MAX_STAGES || colPlan.size() > OWL_MAX_STAGES || rowPlan.empty() || colPlan.empty()) {
+    throw std::invalid_argument("no FFT plan for shape " + fft.spec());
+  }
+  rowT = pow2AtLeast(R * W / 16, 64, 512);
+  while (rowT < 512 && !planFits(rowPlan, W, R, rowT, 16)) { rowT *= 2; }
+  colT = pow2AtLeast(2 * H / 16, 64, 512);
+  while (colT < 512 && !planFits(colPlan, H, 2, colT, 16)) { colT *= 2; }
+  if (!planFits(rowPlan, W, R, rowT, 16) || !planFits(colPlan, H, 2, colT, 16)) {
+    throw std::invalid_argument("FFT plan does not fit the register budget for " + fft.spec());
+  }
+
+  owlCheck(owl_hip_queue_create(&queue_), "queue");
+
+  // ---- host tables (long double, rounded once)
+  const u32 step = stepOf(N, E);
+  const u32 nHi = (H + 63) / 64;
+  std::vector<double2> twRow(W), twCol(H), twMidA(size_t(W) * 64), twMidB(size_t(W) * nHi), twPair(W);
+  for (u32 m = 0; m < W; ++m) { twRow[m] = d2(root1(W, m)); }
+  for (u32 m = 0; m < H; ++m) { twCol[m] = d2(root1(H, m)); }
+  for (u32 a = 0; a < W; ++a) {
+    twPair[a] = d2(root1(ND, a));
+    for (u32 lo = 0; lo < 64; ++lo) { twMidA[size_t(a) * 64 + lo] = d2(root1(ND, u64(a) * lo)); }
+    for (u32 hi = 0; hi < nHi; ++hi) { twMidB[size_t(a) * nHi + hi] = d2(root1(ND, u64(a) * 64 * hi % ND)); }
+  }
+  std::vector<LineW> lineW(H);
+  for (u32 l = 0; l < H; ++l) {
+    for (u32 rep = 0; rep < 2; ++rep) {
+      u32 e = u32(u64(step) * (2 * l + rep) % N);
+      lineW[l].e[rep] = e;
+      lineW[l].w[rep] = double(exp2l((long double)e / N));
+      lineW[l].iw[rep] = double(exp2l(-(long double)e / N));
+    }
+  }
+  std::vector<ColW> colW(W);
+  for (u32 c = 0; c < W; ++c) {
+    u32 e = u32(u64(step) * 2 * H * c % N);
+    colW[c].e = e;
+    colW[c].w = double(exp2l((long double)e / N));
+    colW[c].iw = double(exp2l(-(long double)e / N) / ND);
+    colW[c].pad = 0;
+  }
+
+  JobDesc& J = job_;
+  J.E = E;
+  J.N = N;
+  J.ND = ND;
+  J.W = W;
+  J.H = H;
+  J.R = R;
+  J.nb = nb;
+  J.bitsLo = E / N;
+  J.rE = E % N;
+  J.nHi = nHi;
+  J.nRow = u32(rowPlan.size());
+  J.nCol = u32(colPlan.size());
+  for (u32 i = 0; i < OWL_MAX_STAGES; ++i) {
+    J.rowPlan[i] = i < rowPlan.size() ? rowPlan[i] : 0;
+    J.colPlan[i] = i < colPlan.size() ? colPlan[i] : 0;
+  }
+  J.twRow = upload(twRow);
+  J.twCol = upload(twCol);
+  J.twMidA = upload(twMidA);
+  J.twMidB = upload(twMidB);
+  J.twPair = upload(twPair);
+  J.lineW = upload(lineW);
+  J.colW = upload(colW);
+  for (int i = 0; i < 3; ++i) { J.freq[i] = alloc<double2>(ND); }
+  for (int i = 0; i < 4; ++i) { J.words[i] = alloc<int2>(ND); }
+  for (int i = 0; i < 2; ++i) { J.carry[i] = alloc<int64_t>(size_t(nb) * W); }
+  J.ready = alloc<uint32_t>(nb + 1);
+  J.roe = alloc<uint32_t>(OWL_ROE_SIZE);
+  J.flags = alloc<uint32_t>(8);
+  J.small = alloc<int32_t>(64);
+  J.sum = alloc<uint64_t>(1);
+  owlCheck(owl_hip_fill32(queue_, J.ready, 0, nb + 1), "fill");
+  owlCheck(owl_hip_fill32(queue_, J.roe, 0, OWL_ROE_SIZE), "fill");
+  owlCheck(owl_hip_fill32(queue_, J.flags, 0, 8), "fill");
+  for (int i = 0; i < 4; ++i) { owlCheck(owl_hip_fill32(queue_, J.words[i], 0, 2 * ND), "fill"); }
+  jobs_ = Buffer<JobDesc>(1);
+  jobs_.write(queue_, {J});
+
+  kRow = Kernel("row" + std::to_string(rowT));
+  kCol = Kernel("col" + std::to_string(colT));
+  kCarryB = Kernel("carryB");
+  kCarryFix = Kernel("carryFix");
+  kWordsIn = Kernel("wordsIn");
+  kWordsOut = Kernel("wordsOut");
+  kResidue = Kernel("readResidue");
+  kIsEqual = Kernel("isEqual");
+  kSum64 = Kernel("sum64");
+  kCopy = Kernel("copyWords");
+  finish();
+}
+
+Gpu::~Gpu() {
+  if (queue_) {
+    owl_hip_finish(queue_);
+    owned_.clear();
+    jobs_ = Buffer<JobDesc>();
+    owl_hip_queue_destroy(queue_);
+  }
+}
+
+void Gpu::finish() { owlCheck(owl_hip_finish(queue_), "finish"); }
+
+u32 Gpu::nextRoeSlot() {
+  u32 s = std::min<u32>(roePos, OWL_ROE_SIZE - 1);
+  ++roePos;
+  return s;
+}
+
+// ---------------------------------------------------------------- launches
+void Gpu::rowFused(bool ll, u32 roeSlot) {
+  const JobDesc* j = jobs_.get();
+  int mode = ROW_FUSED, fi = 0, wi = 0, opts = ll ? 1 : 0;
+  kRow(queue_, nb + 1, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot});
+}
+
+void Gpu::rowLeadInTo(Buf in, int freqIdx) {
+  const JobDesc* j = jobs_.get();
+  int mode = ROW_LEADIN, fi = freqIdx, wi = in, opts = 0;
+  u32 slot = OWL_ROE_SIZE;
+  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &slot});
+}
+
+void Gpu::rowLeadIn(Buf in) { rowLeadInTo(in, 0); }
+
+void Gpu::rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot) {
+  const JobDesc* j = jobs_.get();
+  int mode = ROW_CARRYOUT, fi = 0, wi = out, opts = (ll ? 1 : 0) | (mul3 ? 2 : 0);
+  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot});
+}
+
+void Gpu::carryAll(Buf b) {
+  const JobDesc* j = jobs_.get();
+  int wi = b, ci = 0, co = 1;
+  kCarryB(queue_, (nb * W + 255) / 256, 1, 256, 0, {&j, &wi, &ci, &co});
+  kCarryFix(queue_, 1, 1, 64, 0, {&j, &wi, &co});
+}
+
+void Gpu::colSquare() {
+  const JobDesc* j = jobs_.get();
+  int mode = COL_SQUARE, fi = 0, oi = 0;
+  kCol(queue_, W / 2, 1, colT, 2 * H * 16, {&j, &mode, &fi, &oi});
+}
+
+void Gpu::colMul() {
+  const JobDesc* j = jobs_.get();
+  int mode = COL_MUL, fi = 0, oi = 2;
+  kCol(queue_, W / 2, 1, colT, 2 * H * 16, {&j, &mode, &fi, &oi});
+}
+
+void Gpu::colFwdSpec() {
+  const JobDesc* j = jobs_.get();
+  int mode = COL_FWD, fi = 2, oi = 2;
+  kCol(queue_, W / 2, 1, colT, 2 * H * 16, {&j, &mode, &fi, &oi});
+}
+
+// ---------------------------------------------------------------- squaring
+void Gpu::square(Buf out, Buf in, bool leadIn, bool leadOut, bool doMul3, bool doLL) {
+  if (leadIn) { rowLeadIn(in); }
+  colSquare();
+  if (leadOut) {
+    rowCarryOut(out, doLL, doMul3, nextRoeSlot());
+    carryAll(out);
+  } else {
+    rowFused(doLL, nextRoeSlot());
+  }
+}
+
+u32 Gpu::squareLoop(Buf out, Buf in, u32 from, u32 to, bool doTailMul3, bool doLL) {
+  bool leadIn = true;
+  for (u32 k = from; k < to; ++k) {
+    bool leadOut = useLongCarry || k == to - 1;
+    square(out, k == from ? in : out, leadIn, leadOut, doTailMul3 && k == to - 1, doLL);
+    leadIn = leadOut;
+  }
+  return to;
+}
+
+void Gpu::modMul(Buf ioA, Buf inB, bool mul3) {
+  rowLeadInTo(inB, 2);
+  colFwdSpec();
+  rowLeadIn(ioA);
+  colMul();
+  rowCarryOut(ioA, false, mul3, nextRoeSlot());
+  carryAll(ioA);
+}
+
+bool Gpu::doCheck(u32 blockSize) {
+  squareLoop(AUX, CHECK, 0, blockSize, true, false);
+  modMul(CHECK, DATA);
+  return isEqual(CHECK, AUX);
+}
+
+u32 Gpu::flagsRead() {
+  std::vector<u32> f(8);
+  owlCheck(owl_hip_read(queue_, f.data(), job_.flags, 8 * 4, 1), "read flags");
+  return (f[0] ? 1 : 0) | (f[1] ? 2 : 0) | (f[2] ? 4 : 0);
+}
+
+bool Gpu::isEqual(Buf a, Buf b) {
+  owlCheck(owl_hip_fill32(queue_, job_.flags + 1, 0, 1), "fill");
+  const JobDesc* j = jobs_.get();
+  int wa = a, wb = b;
+  kIsEqual(queue_, 256, 1, 256, 0, {&j, &wa, &wb});
+  u32 f = flagsRead();
+  if (f & 4) { throw GpuError("carry hand-off timed out"); }
+  return !(f & 2);
+}
+
+void Gpu::copy(Buf dst, Buf src) {
+  const JobDesc* j = jobs_.get();
+  int d = dst, s = src;
+  kCopy(queue_, 256, 1, 256, 0, {&j, &d, &s});
+}
+
+// ---------------------------------------------------------------- state I/O
+void Gpu::writeDigits(Buf b, const std::vector<int>& digits) {
+  if (digits.size() != N) { throw std::invalid_argument("writeDigits: size"); }
+  Buffer<int32_t> seq(N);
+  owlCheck(owl_hip_write(queue_, seq.get(), digits.data(), N * 4, 0), "write");
+  const JobDesc* j = jobs_.get();
+  int wi = b;
+  int32_t* p = seq.get();
+  kWordsIn(queue_, (ND + 255) / 256, 1, 256, 0, {&j, &wi, &p});
+  finish();
+}
+
+void Gpu::writeIn(Buf b, const Words& words) { writeDigits(b, expandBits(words, N, E)); }
+
+std::vector<int> Gpu::readDigits(Buf b) {
+  Buffer<int32_t> seq(N);
+  const JobDesc* j = jobs_.get();
+  int wi = b;
+  int32_t* p = seq.get();
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    owlCheck(owl_hip_fill32(queue_, job_.sum, 0, 2), "fill");
+    kSum64(queue_, 256, 1, 256, 0, {&j, &wi});
+    kWordsOut(queue_, (ND + 255) / 256, 1, 256, 0, {&j, &wi, &p});
+    std::vector<int> out(N);
+    u64 gpuSum = 0;
+    owlCheck(owl_hip_read(queue_, &gpuSum, job_.sum, 8, 0), "read");
+    owlCheck(owl_hip_read(queue_, out.data(), seq.get(), N * 4, 1), "read");
+    u64 hostSum = 0;
+    for (u32 k = 0; k < N; k += 2) { hostSum += u32(out[k]) | (u64(u32(out[k + 1])) << 32); }
+    if (hostSum == gpuSum) { return out; }
+  }
+  throw GpuError("persistent GPU read errors");
+}
+
+Words Gpu::readData(Buf b) {
+  u32 f = flagsRead();
+  if (f & 4) { throw GpuError("carry hand-off timed out"); }
+  std::vector<int> d = readDigits(b);
+  if (std::all_of(d.begin(), d.end(), [](int x) { return x == 0; })) { return {}; }
+  return compactBits(d, E);
+}
+
+u64 Gpu::dataResidue(Buf b) {
+  const JobDesc* j = jobs_.get();
+  int wi = b;
+  kResidue(queue_, 1, 1, 64, 0, {&j, &wi});
+  int w[64];
+  owlCheck(owl_hip_read(queue_, w, job_.small, sizeof(w), 1), "read residue");
+  u32 f = flagsRead();
+  if (f & 4) { throw GpuError("carry hand-off timed out"); }
+  return residueFromEnds(w, N, E);
+}
+
+RoeInfo Gpu::readROE() {
+  u32 n = std::min<u32>(roePos, OWL_ROE_SIZE);
+  RoeInfo r;
+  if (!n) { return r; }
+  std::vector<uint32_t> bits(n);
+  owlCheck(owl_hip_read(queue_, bits.data(), job_.roe, n * 4, 1), "read roe");
+  owlCheck(owl_hip_fill32(queue_, job_.roe, 0, n), "fill");
+  roePos = 0;
+  double sum = 0, sum2 = 0;
+  for (uint32_t b : bits) {
+    float x;
+    memcpy(&x, &b, 4);
+    r.max = std::max<double>(r.max, x);
+    sum += x;
+    sum2 += double(x) * x;
+  }
+  r.N = n;
+  r.mean = sum / n;
+  r.sd = std::sqrt(std::max(0.0, n * sum2 - sum * sum)) / n;
+  return r;
+}
+
+// ---------------------------------------------------------------- workflows
+namespace {
+double now() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+Words makeWords(u32 E, u32 v) {
+  Words w(nWords(E));
+  w[0] = v;
+  return w;
+}
+}  // namespace
+
+PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* resAt, u32 resStep) {
+  if (!B) { throw std::invalid_argument("block size"); }
+  if (!checkStep) { checkStep = 10 * B; }
+  if (checkStep % B) { throw std::invalid_argument("check step must be a multiple of the block size"); }
+  PRPResult out;
+  Words vData = makeWords(E, 3), vCheck = makeWords(E, 1);
+  u32 vK = 0;
+  double t0 = now();
+  readROE();
+  double maxRoe = 0;
+  while (true) {
+    writeIn(DATA, vData);
+    writeIn(CHECK, vCheck);
+    u32 k = vK;
+    if (resAt && resStep) { resAt->resize(vK / resStep); }
+    bool skip = false, leadIn = true, failed = false;
+    while (k < iters) {
+      if (skip) {
+        skip = false;
+      } else if (k % B == 0) {
+        modMul(CHECK, DATA);
+      }
+      ++k;
+      bool leadOut = (k % B == 0) || k == iters || (resStep && k % resStep == 0) || useLongCarry;
+      square(DATA, DATA, leadIn, leadOut);
+      leadIn = leadOut;
+      if (resAt && resStep && k % resStep == 0) { resAt->push_back(dataResidue()); }
+      bool check = (k % checkStep == 0) || (k == iters && k % B == 0);
+      if (check) {
+        maxRoe = std::max(maxRoe, readROE().max);
+        if (this->doCheck(B)) {
+          ++out.nChecks;
+          skip = true;
+          vData = readData(DATA);
+          vCheck = readData(CHECK);
+          vK = k;
+        } else {
+          ++out.nErrors;
+          if (out.nErrors > 3) { throw GpuError("too many Gerbicz errors"); }
+          failed = true;
+          break;
+        }
+      }
+    }
+    if (failed) { continue; }
+    out.k = k;
+    break;
+  }
+  maxRoe = std::max(maxRoe, readROE().max);
+  out.res64 = dataResidue();
+  if (out.k == E) { out.isPrime = equals9(readData()); }
+  out.maxRoe = maxRoe;
+  out.secsPerIt = (now() - t0) / std::max<u32>(1, iters);
+  return out;
+}
+
+LLResult Gpu::isPrimeLL(u32 iters, std::vector<u64>* resAt, u32 resStep) {
+  LLResult out;
+  writeIn(DATA, makeWords(E, 4));
+  double t0 = now();
+  bool leadIn = true;
+  for (u32 k = 1; k <= iters; ++k) {
+    bool leadOut = (resStep && k % resStep == 0) || k == iters || useLongCarry;
+    square(DATA, DATA, leadIn, leadOut, false, true);
+    leadIn = leadOut;
+    if (resAt && resStep && k % resStep == 0) { resAt->push_back(dataResidue()); }
+  }
+  out.k = iters;
+  Words w = readData();
+  out.res64 = res64(w);
+  out.isPrime = (iters == E - 2) && w.empty();
+  out.secsPerIt = (now() - t0) / std::max<u32>(1, iters);
+  return out;
+}
+
+double Gpu::timePRP(u32 iters, u32 warmup, u32 B) {
+  // Like the reference timePRP (src/Gpu.cpp:1337-1398): includes one Gerbicz update per block.
+  writeIn(DATA, makeWords(E, 3));
+  writeIn(CHECK, makeWords(E, 1));
+  u32 k = 0;
+  modMul(CHECK, DATA);
+  square(DATA, DATA, true, useLongCarry);
+  ++k;
+  while (k < warmup) {
+    square(DATA, DATA, useLongCarry, useLongCarry);
+    ++k;
+  }
+  finish();
+  double t0 = now();
+  bool leadIn = useLongCarry;
+  while (true) {
+    while (k % B < B - 1) {
+      square(DATA, DATA, leadIn, useLongCarry);
+      ++k;
+      leadIn = useLongCarry;
+    }
+    square(DATA, DATA, useLongCarry, true);
+    leadIn = true;
+    ++k;
+    if (k >= iters) { break; }
+    modMul(CHECK, DATA);
+  }
+  finish();
+  double secsPerIt = (now() - t0) / (iters - warmup);
+  if (!doCheck(B)) { return 0.1 * 1e6; }
+  return secsPerIt * 1e6;
+}
+
+double Gpu::timeSquares(u32 iters, u32 warmup, bool useGraph) {
+  writeIn(DATA, makeWords(E, 3));
+  rowLeadIn(DATA);
+  for (u32 i = 0; i < warmup; ++i) {
+    colSquare();
+    rowFused(false, OWL_ROE_SIZE - 1);
+  }
+  owl_graph graph = nullptr;
+  const u32 G = 10;
+  if (useGraph) {
+    owlCheck(owl_hip_graph_begin(queue_), "graph begin");
+    for (u32 i = 0; i < G; ++i) {
+      colSquare();
+      rowFused(false, OWL_ROE_SIZE - 1);
+    }
+    owlCheck(owl_hip_graph_end(queue_, &graph), "graph end");
+    owlCheck(owl_hip_graph_launch(graph, queue_), "graph launch");
+  }
+  finish();
+  owl_event a, b;
+  owlCheck(owl_hip_event_create(&a), "event");
+  owlCheck(owl_hip_event_create(&b), "event");
+  owlCheck(owl_hip_event_record(a, queue_), "event");
+  u32 done = 0;
+  while (done < iters) {
+    if (useGraph) {
+      owlCheck(owl_hip_graph_launch(graph, queue_), "graph launch");
+      done += G;
+    } else {
+      colSquare();
+      rowFused(false, OWL_ROE_SIZE - 1);
+      ++done;
+    }
+  }
+  owlCheck(owl_hip_event_record(b, queue_), "event");
+  float ms = 0;
+  owlCheck(owl_hip_event_elapsed_ms(a, b, &ms), "elapsed");
+  owl_hip_event_destroy(a);
+  owl_hip_event_destroy(b);
+  if (graph) { owl_hip_graph_destroy(graph); }
+  // leave the state consistent: one more squaring ending in a lead-out
+  colSquare();
+  rowCarryOut(DATA, false, false, OWL_ROE_SIZE - 1);
+  carryAll(DATA);
+  finish();
+  return ms * 1e3 / done;
+}

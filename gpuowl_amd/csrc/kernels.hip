@@ -1,0 +1,730 @@
+// HIP kernels for one PRP/LL modular squaring mod 2^E - 1 as an FP64 IBDWT, for gfx950.
+//
+// One steady-state iteration is two in-place passes over the FP64 state:
+//   k_row (ROW_FUSED)  per block of R lines: inverse W-point FFT, unweight, round,
+//                      carry, stairway carry hand-off, weight, forward W-point FFT.
+//                      Replaces the reference carryFused (src/cl/carryfused.cl:21-231).
+//   k_col (COL_SQUARE) per pair of columns: mid twiddle, forward H-point FFT,
+//                      real-signal pairing + square, inverse FFT, mid twiddle.
+//                      Replaces fftMiddleIn + tailSquare + fftMiddleOut
+//                      (src/cl/fftmiddlein.cl:8-48, src/cl/tailsquare.cl:225-313,
+//                      src/cl/fftmiddleout.cl:8-58) in one pass.
+// Boundary kernels: k_row ROW_LEADIN (fftP, src/cl/fftp.cl:10-33), k_row ROW_CARRYOUT
+// (fftW + carryA, src/cl/fftw.cl:9-19, src/cl/carry.cl:9-41), k_carryb (carryB,
+// src/cl/carryb.cl:5-32), k_col COL_MUL / COL_FWD (tailMul, src/cl/tailmul.cl:19-127),
+// k_words_in/out (transposeIn/Out, src/cl/transpose.cl:8-40), k_residue
+// (readResidue, src/cl/etc.cl:9-15), k_is_equal (isEqual, src/cl/etc.cl:34-41),
+// k_sum64 (sum64, src/cl/etc.cl:19-29).
+#include <hip/hip_runtime.h>
+#include "job.h"
+
+template<int R> struct Roots;
+#include "roots_small.h"
+
+typedef double2 T2;
+
+// ---------------------------------------------------------------- complex helpers
+__device__ __forceinline__ T2 cadd(T2 a, T2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ T2 csub(T2 a, T2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ T2 conj2(T2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+// a * conj(b)
+__device__ __forceinline__ T2 cmulc(T2 a, T2 b) {
+  return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
+}
+__device__ __forceinline__ T2 csqr(T2 a) { return make_double2(fma(a.x, a.x, -a.y * a.y), 2 * a.x * a.y); }
+__device__ __forceinline__ T2 cscale(T2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// Multiply by exp(-i pi/2) = -i (forward) or +i (inverse).
+template<bool INV> __device__ __forceinline__ T2 mulMinusI(T2 a) {
+  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+// Multiply by exp(-+2 pi i j / R) with compile-time constants (after unrolling).
+template<int R, bool INV> __device__ __forceinline__ T2 rootMul(T2 a, int j) {
+  j %= R;
+  if (j == 0) { return a; }
+  if (4 * j == R) { return mulMinusI<INV>(a); }
+  if (2 * j == R) { return make_double2(-a.x, -a.y); }
+  if (4 * j == 3 * R) { return mulMinusI<!INV>(a); }
+  double c = Roots<R>::c[j];
+  double s = INV ? Roots<R>::s[j] : -Roots<R>::s[j];
+  return make_double2(fma(a.x, c, -a.y * s), fma(a.x, s, a.y * c));
+}
+
+// ---------------------------------------------------------------- small DFTs
+// Dft<R, INV>::run(a): in-place R-point DFT, natural order in and out,
+// kernel exp(-2 pi i/R) forward and exp(+2 pi i/R) inverse (no scaling).
+template<int R, bool INV> struct Dft;
+
+template<bool INV> struct Dft<2, INV> {
+  __device__ __forceinline__ static void run(T2* a) {
+    T2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template<bool INV> struct Dft<4, INV> {
+  __device__ __forceinline__ static void run(T2* a) {
+    T2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    T2 t2 = cadd(a[1], a[3]), t3 = mulMinusI<INV>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+// Odd prime P: symmetric sums S_k = a_k + a_{P-k}, differences D_k = a_k - a_{P-k}.
+template<int P, bool INV> struct DftOdd {
+  __device__ __forceinline__ static void run(T2* a) {
+    constexpr int h = (P - 1) / 2;
+    T2 S[h], D[h];
+    T2 a0 = a[0];
+    T2 sum = a0;
+#pragma unroll
+    for (int k = 1; k <= h; ++k) {
+      S[k - 1] = cadd(a[k], a[P - k]);
+      D[k - 1] = csub(a[k], a[P - k]);
+      sum = cadd(sum, S[k - 1]);
+    }
+#pragma unroll
+    for (int m = 1; m <= h; ++m) {
+      T2 A = a0;
+      T2 B = make_double2(0, 0);
+#pragma unroll
+      for (int k = 1; k <= h; ++k) {
+        int idx = (k * m) % P;
+        double c = Roots<P>::c[idx], s = Roots<P>::s[idx];
+        A.x = fma(c, S[k - 1].x, A.x);
+        A.y = fma(c, S[k - 1].y, A.y);
+        B.x = fma(s, D[k - 1].x, B.x);
+        B.y = fma(s, D[k - 1].y, B.y);
+      }
+      // forward: X_m = A - iB, X_{P-m} = A + iB; inverse swaps the signs.
+      T2 mIB = make_double2(B.y, -B.x);  // -i*B
+      T2 pIB = make_double2(-B.y, B.x);  // +i*B
+      a[m] = INV ? cadd(A, pIB) : cadd(A, mIB);
+      a[P - m] = INV ? cadd(A, mIB) : cadd(A, pIB);
+    }
+    a[0] = sum;
+  }
+};
+template<bool INV> struct Dft<3, INV> : DftOdd<3, INV> {};
+template<bool INV> struct Dft<5, INV> : DftOdd<5, INV> {};
+template<bool INV> struct Dft<7, INV> : DftOdd<7, INV> {};
+template<bool INV> struct Dft<11, INV> : DftOdd<11, INV> {};
+template<bool INV> struct Dft<13, INV> : DftOdd<13, INV> {};
+
+// Composite R = P*Q by Cooley-Tukey in registers: input index Q*k1 + k2,
+// output index m1 + P*m2, inner twiddle exp(-+2 pi i k2*m1 / R).
+template<int P, int Q, bool INV> struct DftCT {
+  __device__ __forceinline__ static void run(T2* a) {
+    constexpr int R = P * Q;
+    T2 b[Q][P];
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) {
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) { b[k2][k1] = a[Q * k1 + k2]; }
+      Dft<P, INV>::run(b[k2]);
+#pragma unroll
+      for (int m1 = 1; m1 < P; ++m1) { b[k2][m1] = rootMul<R, INV>(b[k2][m1], k2 * m1); }
+    }
+#pragma unroll
+    for (int m1 = 0; m1 < P; ++m1) {
+      T2 c[Q];
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) { c[k2] = b[k2][m1]; }
+      Dft<Q, INV>::run(c);
+#pragma unroll
+      for (int m2 = 0; m2 < Q; ++m2) { a[m1 + P * m2] = c[m2]; }
+    }
+  }
+};
+template<bool INV> struct Dft<6, INV> : DftCT<2, 3, INV> {};
+template<bool INV> struct Dft<8, INV> : DftCT<2, 4, INV> {};
+template<bool INV> struct Dft<9, INV> : DftCT<3, 3, INV> {};
+template<bool INV> struct Dft<10, INV> : DftCT<2, 5, INV> {};
+template<bool INV> struct Dft<12, INV> : DftCT<4, 3, INV> {};
+template<bool INV> struct Dft<14, INV> : DftCT<2, 7, INV> {};
+template<bool INV> struct Dft<15, INV> : DftCT<3, 5, INV> {};
+template<bool INV> struct Dft<16, INV> : DftCT<4, 4, INV> {};
+
+// ---------------------------------------------------------------- LDS Stockham FFT
+// Elements per thread staged in registers during one stage. The host only
+// chooses plans where ceil(L*nfft/R / T) * R <= OWL_EPT for every stage.
+#define OWL_EPT 16
+
+// One radix-R autosort stage over nfft independent L-point FFTs stored
+// back to back in LDS. Ns = product of the radices of the earlier stages.
+// tw[m] = exp(-2 pi i m / L). Not inlined: with every radix inlined into one
+// kernel the register allocator does not share registers between the stages.
+template<int R, bool INV, int T>
+__device__ __noinline__ void fftStage(T2* lds, int L, int nfft, int Ns, const T2* __restrict__ tw, int tid) {
+  constexpr int MB = OWL_EPT / R;
+  const int nbf = L / R;
+  const int total = nbf * nfft;
+  const int stride = L / (Ns * R);
+  T2 v[MB][R];
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    int g = tid + m * T;
+    if (g < total) {
+      int f = g / nbf;
+      int j = g - f * nbf;
+      const T2* x = lds + f * L;
+      int p = j % Ns;
+#pragma unroll
+      for (int k = 0; k < R; ++k) { v[m][k] = x[j + k * nbf]; }
+      if (p) {
+        int st = p * stride;
+#pragma unroll
+        for (int k = 1; k < R; ++k) {
+          T2 w = tw[st * k];
+          v[m][k] = INV ? cmulc(v[m][k], w) : cmul(v[m][k], w);
+        }
+      }
+      Dft<R, INV>::run(v[m]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    int g = tid + m * T;
+    if (g < total) {
+      int f = g / nbf;
+      int j = g - f * nbf;
+      T2* x = lds + f * L;
+      int p = j % Ns;
+      int base = (j - p) * R + p;
+#pragma unroll
+      for (int k = 0; k < R; ++k) { x[base + k * Ns] = v[m][k]; }
+    }
+  }
+  __syncthreads();
+}
+
+// Full mixed-radix FFT; the caller must have synchronized after filling LDS.
+template<bool INV, int T>
+__device__ void fftLds(T2* lds, int L, int nfft, const uint32_t* plan, int nStages, const T2* __restrict__ tw, int tid) {
+  int Ns = 1;
+  for (int s = 0; s < nStages; ++s) {
+    int R = plan[s];
+    switch (R) {
+      case 2: fftStage<2, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 3: fftStage<3, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 4: fftStage<4, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 5: fftStage<5, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 6: fftStage<6, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 7: fftStage<7, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 8: fftStage<8, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 9: fftStage<9, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 10: fftStage<10, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 11: fftStage<11, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 12: fftStage<12, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 13: fftStage<13, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 14: fftStage<14, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 15: fftStage<15, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      case 16: fftStage<16, INV, T>(lds, L, nfft, Ns, tw, tid); break;
+      default: break;  // the host never builds such a plan
+    }
+    Ns *= R;
+  }
+}
+
+// ---------------------------------------------------------------- word helpers
+// Rounding constant 3*2^51: for |v| < 2^51, v + RNDVAL holds rint(v) in its low mantissa bits.
+#define OWL_RNDVAL 6755399441055744.0
+#define OWL_RNDVAL_BITS 0x4338000000000000LL
+
+__device__ __forceinline__ int64_t lowBits64(int64_t u, int b) { return (int64_t)((uint64_t)u << (64 - b)) >> (64 - b); }
+
+// Slot s of a frequency row -> frequency index a (see job.h).
+__device__ __forceinline__ int slotFreq(int s, int W) {
+  int q = s >> 1;
+  return (s & 1) ? (q ? W - q : W / 2) : q;
+}
+
+// Weight data for word (line, col, rep).
+struct WordW {
+  double w, iw;
+  int bits;
+};
+__device__ __forceinline__ WordW wordWeight(const JobDesc& J, const LineW& lw, const ColW& cw, int rep) {
+  uint32_t e = lw.e[rep] + cw.e;
+  bool wrap = e >= J.N;
+  uint32_t ex = wrap ? e - J.N : e;
+  WordW r;
+  r.bits = J.bitsLo + (ex < J.rE);
+  r.w = lw.w[rep] * cw.w;
+  r.iw = lw.iw[rep] * cw.iw;
+  if (wrap) {
+    r.w *= 0.5;
+    r.iw *= 2.0;
+  }
+  return r;
+}
+
+// Round v to the nearest integer (|v| < 2^51); track the round-off.
+__device__ __forceinline__ int64_t roundWord(double v, float& roe) {
+  double t = v + OWL_RNDVAL;
+  double r = t - OWL_RNDVAL;
+  roe = fmaxf(roe, (float)fabs(v - r));
+  return __double_as_longlong(t) - OWL_RNDVAL_BITS;
+}
+
+__device__ __forceinline__ void roeReport(const JobDesc& J, uint32_t slot, float roe) {
+  // wave-level max, then one atomic per wave (float bits of non-negative values order as uints)
+  for (int off = 32; off > 0; off >>= 1) { roe = fmaxf(roe, __shfl_xor(roe, off, 64)); }
+  if ((threadIdx.x & 63) == 0 && slot < OWL_ROE_SIZE) { atomicMax(J.roe + slot, __float_as_uint(roe)); }
+}
+
+#define OWL_SPIN_LIMIT (1u << 24)
+
+// ---------------------------------------------------------------- row pass
+// Grid (nb + 1, batch) for ROW_FUSED, (nb, batch) otherwise; LDS R*W*16 bytes.
+// opts: bit0 = LL (-2 as carry-in of word 0), bit1 = MUL3 (x3 before the carry).
+// ROW_FUSED: freq[fi] -> freq[fi] in place (group nb redoes block 0 with the
+//   wrap-around carry, as the reference carryFused does with group H; ready[]
+//   has nb + 1 flags).
+// ROW_LEADIN: words[wi] -> freq[fi].
+// ROW_CARRYOUT: freq[fi] -> words[wi] + carry[0] (block-normalized words).
+template<int T>
+__global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts, uint32_t roeSlot) {
+  extern __shared__ T2 lds[];
+  const JobDesc& J = jobs[blockIdx.y];
+  const int W = J.W, R = J.R, nb = J.nb;
+  const int g = blockIdx.x;
+  const int lb = (g == nb) ? 0 : g;
+  const int tid = threadIdx.x;
+  const int LW = R * W;
+  T2* blk = J.freq[fi] + (size_t)lb * LW;
+  int2* ws = (int2*)lds;  // digit view of an LDS slot (thread-private during the carry phase)
+  const bool ll = opts & 1, mul3 = opts & 2;
+
+  if (mode == ROW_LEADIN) {
+    const int2* src = J.words[wi] + (size_t)lb * LW;
+    for (int c = tid; c < W; c += T) {
+      for (int r = 0; r < R; ++r) { ws[2 * (r * W + c)] = src[c * R + r]; }
+    }
+  } else {
+    for (int e = tid; e < LW; e += T) {
+      int s = e / R;
+      int r = e - s * R;
+      lds[r * W + slotFreq(s, W)] = blk[e];
+    }
+    __syncthreads();
+    fftLds<true, T>(lds, W, R, J.rowPlan, J.nRow, J.twRow, tid);
+
+    float roe = 0;
+    for (int c = tid; c < W; c += T) {
+      const ColW cw = J.colW[c];
+      int64_t s = 0;
+      for (int r = 0; r < R; ++r) {
+        const int l = lb * R + r;
+        const LineW lw = J.lineW[l];
+        T2 z = lds[r * W + c];
+        int32_t d[2];
+#pragma unroll
+        for (int rep = 0; rep < 2; ++rep) {
+          WordW ww = wordWeight(J, lw, cw, rep);
+          int64_t u = roundWord((rep ? z.y : z.x) * ww.iw, roe);
+          if (mul3) { u *= 3; }
+          if (ll && l == 0 && c == 0 && rep == 0) { u -= 2; }
+          u += s;
+          int64_t dd = lowBits64(u, ww.bits);
+          s = (u - dd) >> ww.bits;
+          d[rep] = (int32_t)dd;
+        }
+        ws[2 * (r * W + c)] = make_int2(d[0], d[1]);
+      }
+      if (mode == ROW_CARRYOUT) {
+        J.carry[0][(size_t)lb * W + c] = s;
+      } else if (g < nb) {
+        __hip_atomic_store(J.carry[0] + (size_t)g * W + c, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    roeReport(J, roeSlot, roe);
+
+    if (mode == ROW_CARRYOUT) {
+      int2* dst = J.words[wi] + (size_t)lb * LW;
+      for (int c = tid; c < W; c += T) {
+        for (int r = 0; r < R; ++r) { dst[c * R + r] = ws[2 * (r * W + c)]; }
+      }
+      return;
+    }
+
+    // Stairway: publish this block's carries, then take the previous block's.
+    // Group 0 also raises ready[nb] once it is done with block 0 (its carries
+    // depend on every value it read), because group nb rewrites block 0 in place.
+    if (g < nb) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (tid == 0) {
+        __hip_atomic_store(J.ready + g, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (g == 0) { __hip_atomic_store(J.ready + nb, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
+      }
+    }
+    if (g == 0) { return; }
+    const int pred = g - 1;
+    if (tid == 0) {
+      for (int w = 0; w < (g == nb ? 2 : 1); ++w) {
+        uint32_t* flag = J.ready + (w ? nb : pred);
+        uint32_t n = 0;
+        while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++n >= OWL_SPIN_LIMIT) {
+            atomicOr(J.flags + 2, 1u);  // hand-off timed out: flagged to the host, kernel still exits
+            break;
+          }
+        }
+        __hip_atomic_store(flag, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int c = tid; c < W; c += T) {
+      // block 0 takes the carry of the last block of the previous column (word order
+      // wraps from column W-1 to column 0 because 2^E == 1 mod 2^E - 1)
+      int srcCol = (g == nb) ? (c == 0 ? W - 1 : c - 1) : c;
+      int64_t carry = __hip_atomic_load(J.carry[0] + (size_t)pred * W + srcCol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // Ripple the incoming carry through this block's 2R words; what survives the
+      // block (rare: the host picks R so the carry dies well inside it) stays in the
+      // last word. Leaving it all in the second word, as a one-step hand-off does,
+      // makes the iteration diverge below about 12 bits per word.
+      const ColW cw = J.colW[c];
+      for (int r = 0; r < R && carry; ++r) {
+        const LineW lw = J.lineW[lb * R + r];
+        int2 d = ws[2 * (r * W + c)];
+        WordW w0 = wordWeight(J, lw, cw, 0);
+        int64_t t = (int64_t)d.x + carry;
+        int64_t nd = lowBits64(t, w0.bits);
+        carry = (t - nd) >> w0.bits;
+        d.x = (int32_t)nd;
+        if (r == R - 1) {
+          d.y += (int32_t)carry;
+          carry = 0;
+        } else if (carry) {
+          WordW w1 = wordWeight(J, lw, cw, 1);
+          t = (int64_t)d.y + carry;
+          nd = lowBits64(t, w1.bits);
+          carry = (t - nd) >> w1.bits;
+          d.y = (int32_t)nd;
+        }
+        ws[2 * (r * W + c)] = d;
+      }
+    }
+  }
+
+  // Weight the digits and run the forward row FFT.
+  for (int c = tid; c < W; c += T) {
+    const ColW cw = J.colW[c];
+    for (int r = 0; r < R; ++r) {
+      const LineW lw = J.lineW[lb * R + r];
+      int2 d = ws[2 * (r * W + c)];
+      WordW w0 = wordWeight(J, lw, cw, 0);
+      WordW w1 = wordWeight(J, lw, cw, 1);
+      lds[r * W + c] = make_double2(d.x * w0.w, d.y * w1.w);
+    }
+  }
+  __syncthreads();
+  fftLds<false, T>(lds, W, R, J.rowPlan, J.nRow, J.twRow, tid);
+  for (int e = tid; e < LW; e += T) {
+    int s = e / R;
+    int r = e - s * R;
+    blk[e] = lds[r * W + slotFreq(s, W)];
+  }
+}
+
+// ---------------------------------------------------------------- column pass
+__device__ __forceinline__ T2 midTw(const JobDesc& J, int a, int l) {
+  return cmul(J.twMidA[a * 64 + (l & 63)], J.twMidB[a * J.nHi + (l >> 6)]);
+}
+
+// Real-signal pairing for the squaring (see the derivation in the host docs):
+// with D = X - conj(Xp), t = (1 + w) D^2 / 4:  X <- X^2 - t,  Xp <- Xp^2 - conj(t),
+// where w = exp(-2 pi i K / ND) for X = Z_K and Xp = Z_{ND-K}.
+__device__ __forceinline__ void pairSq(T2& X, T2& Xp, T2 w) {
+  T2 D = csub(X, conj2(Xp));
+  T2 t = cscale(cmul(make_double2(1.0 + w.x, w.y), csqr(D)), 0.25);
+  T2 nX = csub(csqr(X), t);
+  Xp = csub(csqr(Xp), conj2(t));
+  X = nX;
+}
+// Same for a product with a second transformed operand (Y, Yp).
+__device__ __forceinline__ void pairMul(T2& X, T2& Xp, T2 Y, T2 Yp, T2 w) {
+  T2 D = csub(X, conj2(Xp));
+  T2 E = csub(Y, conj2(Yp));
+  T2 t = cscale(cmul(make_double2(1.0 + w.x, w.y), cmul(D, E)), 0.25);
+  T2 nX = csub(cmul(X, Y), t);
+  Xp = csub(cmul(Xp, Yp), conj2(t));
+  X = nX;
+}
+
+// Grid (W/2, batch); LDS 2*H*16 bytes. Group q owns slots 2q and 2q+1.
+// COL_SQUARE: freq[fi] <- square, in place.   COL_MUL: freq[fi] <- freq[fi] * spectrum freq[oi].
+// COL_FWD: freq[fi] <- forward column transform only (the spectrum of a second operand).
+template<int T>
+__global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int mode, int fi, int oi) {
+  extern __shared__ T2 lds[];
+  const JobDesc& J = jobs[blockIdx.y];
+  const int W = J.W, H = J.H, R = J.R;
+  const int q = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int a0 = q, a1 = q ? W - q : W / 2;
+  T2* io = J.freq[fi];
+  const int per = 2 * R;
+  const size_t colBase = (size_t)2 * q * R;
+
+  for (int e = tid; e < 2 * H; e += T) {
+    int lb = e / per;
+    int wi = e - lb * per;
+    int h = wi / R;
+    int l = lb * R + wi - h * R;
+    T2 v = io[(size_t)lb * W * R + colBase + wi];
+    lds[h * H + l] = cmul(v, midTw(J, h ? a1 : a0, l));
+  }
+  __syncthreads();
+  fftLds<false, T>(lds, H, 2, J.colPlan, J.nCol, J.twCol, tid);
+
+  if (mode == COL_FWD) {
+    for (int e = tid; e < 2 * H; e += T) {
+      int lb = e / per;
+      int wi = e - lb * per;
+      int h = wi / R;
+      int b = lb * R + wi - h * R;
+      io[(size_t)lb * W * R + colBase + wi] = lds[h * H + b];
+    }
+    return;
+  }
+
+  const T2* other = (mode == COL_MUL) ? J.freq[oi] : nullptr;
+  // position of spectrum bin b of half h in a COL_FWD output buffer
+  auto spos = [&](int h, int b) -> size_t { return (size_t)(b / R) * W * R + colBase + h * R + (b % R); };
+
+  if (q) {
+    const T2 wa = J.twPair[a0];
+    for (int b = tid; b < H; b += T) {
+      int bp = H - 1 - b;
+      T2 X = lds[b], Xp = lds[H + bp];
+      T2 w = cmul(wa, J.twCol[b]);
+      if (other) {
+        pairMul(X, Xp, other[spos(0, b)], other[spos(1, bp)], w);
+      } else {
+        pairSq(X, Xp, w);
+      }
+      lds[b] = X;
+      lds[H + bp] = Xp;
+    }
+  } else {
+    // column 0 pairs b with H-b (b = 0 and b = H/2 pair with themselves);
+    // column W/2 pairs b with H-1-b.
+    const T2 wa = J.twPair[W / 2];
+    for (int i = tid; i <= H; i += T) {
+      if (i <= H / 2) {
+        int b = i, bp = (H - i) % H;
+        T2 X = lds[b], Xp = lds[bp];
+        T2 w = J.twCol[b];
+        if (other) {
+          pairMul(X, Xp, other[spos(0, b)], other[spos(0, bp)], w);
+        } else {
+          pairSq(X, Xp, w);
+        }
+        lds[b] = X;
+        if (bp != b) { lds[bp] = Xp; }
+      } else {
+        int b = i - H / 2 - 1, bp = H - 1 - b;
+        T2 X = lds[H + b], Xp = lds[H + bp];
+        T2 w = cmul(wa, J.twCol[b]);
+        if (other) {
+          pairMul(X, Xp, other[spos(1, b)], other[spos(1, bp)], w);
+        } else {
+          pairSq(X, Xp, w);
+        }
+        lds[H + b] = X;
+        lds[H + bp] = Xp;
+      }
+    }
+  }
+  __syncthreads();
+  fftLds<true, T>(lds, H, 2, J.colPlan, J.nCol, J.twCol, tid);
+
+  for (int e = tid; e < 2 * H; e += T) {
+    int lb = e / per;
+    int wi = e - lb * per;
+    int h = wi / R;
+    int l = lb * R + wi - h * R;
+    io[(size_t)lb * W * R + colBase + wi] = cmulc(lds[h * H + l], midTw(J, h ? a1 : a0, l));
+  }
+}
+
+// ---------------------------------------------------------------- carry propagation
+// One thread per (line block, column): add the carry-out of the preceding block
+// (in word order) and propagate through this block's 2R words. A carry still
+// left after the block goes to carry[co] and raises flags[0]; the host repeats
+// with the buffers swapped until no carry is left.
+__global__ __launch_bounds__(256) void k_carryb(const JobDesc* __restrict__ jobs, int wi, int ci, int co) {
+  const JobDesc& J = jobs[blockIdx.y];
+  const int W = J.W, R = J.R, nb = J.nb;
+  int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= nb * W) { return; }
+  int lb = idx / W, c = idx - lb * W;
+  int plb = lb ? lb - 1 : nb - 1;
+  int pc = lb ? c : (c ? c - 1 : W - 1);
+  int64_t carry = J.carry[ci][(size_t)plb * W + pc];
+  if (carry) {
+    int2* blk = J.words[wi] + (size_t)lb * W * R + (size_t)c * R;
+    const ColW cw = J.colW[c];
+    for (int r = 0; r < R && carry; ++r) {
+      const LineW lw = J.lineW[lb * R + r];
+      int2 d = blk[r];
+      WordW w0 = wordWeight(J, lw, cw, 0);
+      int64_t t = (int64_t)d.x + carry;
+      int64_t nd = lowBits64(t, w0.bits);
+      carry = (t - nd) >> w0.bits;
+      d.x = (int32_t)nd;
+      if (carry) {
+        WordW w1 = wordWeight(J, lw, cw, 1);
+        t = (int64_t)d.y + carry;
+        nd = lowBits64(t, w1.bits);
+        carry = (t - nd) >> w1.bits;
+        d.y = (int32_t)nd;
+      }
+      blk[r] = d;
+    }
+  }
+  J.carry[co][(size_t)lb * W + c] = carry;
+  if (carry) { atomicOr(J.flags, 1u); }
+}
+
+// Completes, in word order and on one thread, the rare carries that k_carryb could not
+// absorb inside their block (flags[0] set). Exits at once when there are none.
+__global__ __launch_bounds__(64) void k_carry_fix(const JobDesc* __restrict__ jobs, int wi, int ci) {
+  const JobDesc& J = jobs[blockIdx.y];
+  if (threadIdx.x != 0) { return; }
+  if (__hip_atomic_load(J.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) { return; }
+  J.flags[0] = 0;
+  const uint32_t W = J.W, R = J.R, nb = J.nb;
+  const uint32_t total = nb * W;  // blocks in word order: rank = c * nb + lb
+  for (uint32_t t = 0; t < total; ++t) {
+    uint32_t lb = t % nb, c = t / nb;
+    int64_t carry = J.carry[ci][(size_t)lb * W + c];
+    if (!carry) { continue; }
+    J.carry[ci][(size_t)lb * W + c] = 0;
+    for (uint32_t u = (t + 1) % total, n = 0; carry && n < total; u = (u + 1) % total, ++n) {
+      uint32_t ulb = u % nb, uc = u / nb;
+      int2* blk = J.words[wi] + (size_t)ulb * W * R + (size_t)uc * R;
+      const ColW cw = J.colW[uc];
+      for (uint32_t r = 0; r < R && carry; ++r) {
+        const LineW lw = J.lineW[ulb * R + r];
+        int2 d = blk[r];
+        WordW w0 = wordWeight(J, lw, cw, 0);
+        int64_t x = (int64_t)d.x + carry;
+        int64_t nd = lowBits64(x, w0.bits);
+        carry = (x - nd) >> w0.bits;
+        d.x = (int32_t)nd;
+        if (carry) {
+          WordW w1 = wordWeight(J, lw, cw, 1);
+          x = (int64_t)d.y + carry;
+          nd = lowBits64(x, w1.bits);
+          carry = (x - nd) >> w1.bits;
+          d.y = (int32_t)nd;
+        }
+        blk[r] = d;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- state I/O
+// Sequential words (word k at seq[k]) <-> line-block layout of int2 word pairs.
+__global__ __launch_bounds__(256) void k_words_in(const JobDesc* __restrict__ jobs, int wi, const int32_t* __restrict__ seq) {
+  const JobDesc& J = jobs[blockIdx.y];
+  uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= J.ND) { return; }
+  uint32_t l = j % J.H, c = j / J.H;
+  J.words[wi][(size_t)(l / J.R) * J.W * J.R + (size_t)c * J.R + l % J.R] = make_int2(seq[2 * j], seq[2 * j + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_words_out(const JobDesc* __restrict__ jobs, int wi, int32_t* __restrict__ seq) {
+  const JobDesc& J = jobs[blockIdx.y];
+  uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= J.ND) { return; }
+  uint32_t l = j % J.H, c = j / J.H;
+  int2 v = J.words[wi][(size_t)(l / J.R) * J.W * J.R + (size_t)c * J.R + l % J.R];
+  seq[2 * j] = v.x;
+  seq[2 * j + 1] = v.y;
+}
+
+// small[0..31] = the top 32 words (k = N-32 .. N-1), small[32..63] = the bottom 32 words.
+__global__ __launch_bounds__(64) void k_residue(const JobDesc* __restrict__ jobs, int wi) {
+  const JobDesc& J = jobs[blockIdx.y];
+  int t = threadIdx.x;
+  if (t >= 32) { return; }
+  uint32_t j = (t < 16) ? J.ND - 16 + t : t - 16;
+  uint32_t l = j % J.H, c = j / J.H;
+  int2 v = J.words[wi][(size_t)(l / J.R) * J.W * J.R + (size_t)c * J.R + l % J.R];
+  J.small[2 * t] = v.x;
+  J.small[2 * t + 1] = v.y;
+}
+
+__global__ __launch_bounds__(256) void k_is_equal(const JobDesc* __restrict__ jobs, int wa, int wb) {
+  const JobDesc& J = jobs[blockIdx.y];
+  const int64_t* a = (const int64_t*)J.words[wa];
+  const int64_t* b = (const int64_t*)J.words[wb];
+  bool diff = false;
+  for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < J.ND; p += gridDim.x * 256) { diff |= a[p] != b[p]; }
+  if (diff) { atomicOr(J.flags + 1, 1u); }
+}
+
+// 64-bit sum of a words buffer, for verified reads (src/Gpu.cpp:677-705).
+__global__ __launch_bounds__(256) void k_sum64(const JobDesc* __restrict__ jobs, int wi) {
+  const JobDesc& J = jobs[blockIdx.y];
+  const uint64_t* a = (const uint64_t*)J.words[wi];
+  uint64_t s = 0;
+  for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < J.ND; p += gridDim.x * 256) { s += a[p]; }
+  for (int off = 32; off > 0; off >>= 1) { s += __shfl_xor(s, off, 64); }
+  if ((threadIdx.x & 63) == 0) { atomicAdd((unsigned long long*)J.sum, (unsigned long long)s); }
+}
+
+// Fill n u32 with v (used instead of memset so that it can sit inside a captured graph).
+__global__ __launch_bounds__(256) void k_fill32(uint32_t* p, uint32_t n, uint32_t v) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) { p[i] = v; }
+}
+
+__global__ __launch_bounds__(256) void k_copy_words(const JobDesc* __restrict__ jobs, int dst, int src) {
+  const JobDesc& J = jobs[blockIdx.y];
+  for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < J.ND; p += gridDim.x * 256) { J.words[dst][p] = J.words[src][p]; }
+}
+
+// ---------------------------------------------------------------- kernel table
+// Looked up by name through owl_hip_kernel_get (the replacement of the reference's
+// named kernel loading, src/clwrap.cpp:278-284).
+struct OwlKernelEntry {
+  const char* name;
+  const void* fn;
+  unsigned maxThreads;
+};
+
+extern "C" const OwlKernelEntry owl_kernel_table[] = {
+    {"row64", (const void*)&k_row<64>, 64},
+    {"row128", (const void*)&k_row<128>, 128},
+    {"row256", (const void*)&k_row<256>, 256},
+    {"row512", (const void*)&k_row<512>, 512},
+    {"col64", (const void*)&k_col<64>, 64},
+    {"col128", (const void*)&k_col<128>, 128},
+    {"col256", (const void*)&k_col<256>, 256},
+    {"col512", (const void*)&k_col<512>, 512},
+    {"carryB", (const void*)&k_carryb, 256},
+    {"carryFix", (const void*)&k_carry_fix, 64},
+    {"wordsIn", (const void*)&k_words_in, 256},
+    {"wordsOut", (const void*)&k_words_out, 256},
+    {"readResidue", (const void*)&k_residue, 64},
+    {"isEqual", (const void*)&k_is_equal, 256},
+    {"sum64", (const void*)&k_sum64, 256},
+    {"fill32", (const void*)&k_fill32, 256},
+    {"copyWords", (const void*)&k_copy_words, 256},
+    {nullptr, nullptr, 0},
+};

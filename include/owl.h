@@ -1,0 +1,114 @@
+/* owl.h -- C-ABI of the modular-squaring engine (one PRP/LL iteration mod 2^E - 1
+ * as an FP64 IBDWT on gfx950).
+ *
+ * Each entry point wraps a method of the reference's host class Gpu
+ * (src/Gpu.h:213-300) or FFTConfig (src/FFTConfig.h:69-87), cited per function.
+ * Plain pointers and sizes only. Returns 0 on success, non-zero on failure with
+ * the message in owl_last_error() (declared in owl_hip.h). Buffers are named by
+ * index: 0 = data, 1 = aux, 2 = check, 3 = base (src/Gpu.h:134-140).
+ */
+#ifndef OWL_H
+#define OWL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "owl_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct owl_gpu_s* owl_gpu;
+
+typedef struct {
+  uint32_t E, N, W, H, R, nb, rowThreads, colThreads;
+  uint32_t maxExp;
+  int longCarry;
+  char spec[64];     /* reference FFT spec, e.g. "1K:15:256:202" */
+} owl_gpu_info_t;
+
+typedef struct {
+  uint64_t res64;
+  uint32_t k, nErrors, nChecks;
+  int isPrime;
+  double maxRoe, secsPerIt;
+} owl_prp_result_t;
+
+typedef struct {
+  uint64_t res64;
+  uint32_t k;
+  int isPrime;
+  double secsPerIt;
+} owl_ll_result_t;
+
+/* FFTConfig::bestFit (src/FFTConfig.cpp:235-263), without a tune file. spec may be "" or NULL. */
+int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen, uint32_t* N, uint32_t* maxExp);
+
+/* Gpu::Gpu / Gpu::make (src/Gpu.cpp:380-564). lineBlock 0 = automatic. */
+int owl_gpu_create(uint32_t E, const char* spec, int longCarry, uint32_t lineBlock, owl_gpu* gpu);
+int owl_gpu_destroy(owl_gpu gpu);
+int owl_gpu_info(owl_gpu gpu, owl_gpu_info_t* info);
+
+/* Gpu::writeIn / readData (src/Gpu.cpp:801-811, 707-709): packed residue words, (E-1)/32+1 u32.
+ * read: *isZero = 1 and no words written when the residue is all zero. */
+int owl_gpu_write_words(owl_gpu gpu, int buf, const uint32_t* words, uint32_t nWords);
+int owl_gpu_read_words(owl_gpu gpu, int buf, uint32_t* words, uint32_t nWords, int* isZero);
+/* Raw balanced digits in sequential word order, N int32 (src/Gpu.cpp:677-705, 801-811). */
+int owl_gpu_write_digits(owl_gpu gpu, int buf, const int32_t* digits, uint32_t n);
+int owl_gpu_read_digits(owl_gpu gpu, int buf, int32_t* digits, uint32_t n);
+
+/* Gpu::dataResidue / bufResidue (src/Gpu.cpp:955-975). */
+int owl_gpu_res64(owl_gpu gpu, int buf, uint64_t* res);
+
+/* Gpu::squareLoop (src/Gpu.cpp:936-945): iterations [from, to); mul3Last multiplies the last by 3;
+ * ll selects x -> x^2 - 2. */
+int owl_gpu_square_loop(owl_gpu gpu, int out, int in, uint32_t from, uint32_t to, int mul3Last, int ll);
+/* Gpu::modMul (src/Gpu.cpp:729-733): ioA <- ioA * inB (* 3 if mul3). */
+int owl_gpu_mod_mul(owl_gpu gpu, int ioA, int inB, int mul3);
+/* Gpu::doCheck (src/Gpu.cpp:767-771) and Gpu::isEqual (src/Gpu.cpp:947-953). */
+int owl_gpu_do_check(owl_gpu gpu, uint32_t blockSize, int* ok);
+int owl_gpu_is_equal(owl_gpu gpu, int a, int b, int* equal);
+/* Buffer copy (bufAux << bufData, src/Gpu.cpp:741-742). */
+int owl_gpu_copy(owl_gpu gpu, int dst, int src);
+/* Gpu::readROE (src/Gpu.cpp:624-637): statistics of the max round-off per iteration since the last read. */
+int owl_gpu_read_roe(owl_gpu gpu, uint32_t* n, double* maxRoe, double* mean, double* sd);
+
+/* Gpu::isPrimePRP (src/Gpu.cpp:1400-1584) without savefiles or proofs: iters PRP-3 iterations
+ * from x0 = 3 with a Gerbicz check every checkStep (multiple of blockSize). res64 after every
+ * resStep iterations goes to res[0..resCap). */
+int owl_gpu_prp(owl_gpu gpu, uint32_t iters, uint32_t blockSize, uint32_t checkStep, uint64_t* res, uint32_t resStep,
+                uint32_t resCap, owl_prp_result_t* out);
+/* Gpu::isPrimeLL (src/Gpu.cpp:1586-1663) without savefiles: iters LL iterations from x0 = 4. */
+int owl_gpu_ll(owl_gpu gpu, uint32_t iters, uint64_t* res, uint32_t resStep, uint32_t resCap, owl_ll_result_t* out);
+
+/* Gpu::timePRP (src/Gpu.cpp:1337-1398): microseconds per PRP iteration incl. Gerbicz updates. */
+int owl_gpu_time_prp(owl_gpu gpu, uint32_t iters, uint32_t warmup, uint32_t blockSize, double* usPerIt);
+/* Steady-state squarings only (no reference counterpart): us per iteration, optionally replayed from a
+ * captured graph. */
+int owl_gpu_time_squares(owl_gpu gpu, uint32_t iters, uint32_t warmup, int useGraph, double* usPerIt);
+/* Per-kernel average durations over iters steady-state iterations, timed with events on the engine's
+ * queue around each launch (no reference counterpart; the reference's -time profile is src/Gpu.cpp:773-799). */
+int owl_gpu_time_kernels(owl_gpu gpu, uint32_t iters, double* usCol, double* usRow);
+
+/* Kernel-level access for numerics tests (no reference counterpart; the reference's
+ * closest tools are its -verbose self-tests, src/Gpu.cpp:1059-1093).
+ * owl_gpu_freq_io: read (write = 0) or write the FP64 state buffer idx (0 = state,
+ * 2 = second-operand spectrum) as 2*N/2 doubles (re, im) in the engine's HBM layout.
+ * owl_gpu_pass: run one pass on the queue and wait:
+ *   OWL_PASS_LEADIN   words[arg] -> state (weight + forward row FFT)
+ *   OWL_PASS_SQUARE   state <- column pass with the squaring
+ *   OWL_PASS_FUSED    state <- row pass with the fused carry (arg = 1: LL)
+ *   OWL_PASS_CARRYOUT state -> words[arg], fully carried
+ *   OWL_PASS_SPECTRUM spectrum buffer <- forward column transform of itself
+ *   OWL_PASS_MUL      state <- column pass multiplying by the spectrum buffer */
+enum { OWL_PASS_LEADIN = 0, OWL_PASS_SQUARE = 1, OWL_PASS_FUSED = 2, OWL_PASS_CARRYOUT = 3, OWL_PASS_SPECTRUM = 4,
+       OWL_PASS_MUL = 5 };
+int owl_gpu_freq_io(owl_gpu gpu, int idx, double* host, uint32_t nDoubles, int write);
+int owl_gpu_pass(owl_gpu gpu, int pass, int arg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,101 @@
+"""Writes the res64 golden fixtures of tests/golden/ with the GMP oracle
+(oracle/_build/gmp_loop), one process per job, in parallel.
+
+Each fixture res_<name>.txt holds a header line and "k res64" lines:
+    # E=<E> kind=<prp|ll> iters=<n> step=<s>
+    1000 0123456789abcdef
+The exponents are the workloads of the benchmark configurations:
+  A  p = 4,000,037 (smallest prime >= 4,000,000), PRP-3
+  B  p = 139,000,003 (smallest prime >= 139,000,000), PRP-3 and LL
+  C  the first 64 primes >= 20,000,000, PRP-3
+Usage: python tests/golden/make_goldens.py [--jobs N] [names...]"""
+import argparse, os, subprocess, sys, concurrent.futures as cf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+GMP = os.path.join(ROOT, "oracle", "_build", "gmp_loop")
+
+
+def is_prime(n):
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def next_prime(n):
+    while not is_prime(n):
+        n += 1
+    return n
+
+
+def primes_from(n, count):
+    out = []
+    p = n
+    while len(out) < count:
+        p = next_prime(p)
+        out.append(p)
+        p += 1
+    return out
+
+
+def jobs():
+    J = {}
+    A = next_prime(4_000_000)
+    B = next_prime(139_000_000)
+    J["A_prp"] = (A, 0, 100_000, 1000)
+    J["B_prp"] = (B, 0, 20_000, 1000)
+    J["B_ll"] = (B, 1, 20_000, 1000)
+    J["B_prp_short"] = (B, 0, 200, 100)
+    J["B_ll_short"] = (B, 1, 200, 100)
+    for i, p in enumerate(primes_from(20_000_000, 64)):
+        J[f"C{i:02d}_prp"] = (p, 0, 1000, 500)
+    return J
+
+
+def run(name, E, ll, iters, step):
+    path = os.path.join(HERE, f"res_{name}.txt")
+    tmp = path + ".part"
+    with open(tmp, "w") as f:
+        f.write(f"# E={E} kind={'ll' if ll else 'prp'} iters={iters} step={step}\n")
+        f.flush()
+        p = subprocess.Popen([GMP, str(E), str(iters), str(ll), str(step)], stdout=subprocess.PIPE, text=True)
+        for line in p.stdout:
+            if line.startswith("{"):
+                continue
+            f.write(line)
+            f.flush()
+        rc = p.wait()
+    if rc:
+        raise RuntimeError(f"{name}: gmp_loop exit {rc}")
+    os.replace(tmp, path)
+    return name
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("names", nargs="*")
+    a = ap.parse_args()
+    all_jobs = jobs()
+    names = a.names or list(all_jobs)
+    with cf.ThreadPoolExecutor(max_workers=a.jobs) as ex:
+        futs = [ex.submit(run, n, *all_jobs[n]) for n in names]
+        for f in cf.as_completed(futs):
+            print("done", f.result(), flush=True)

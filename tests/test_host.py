@@ -1,0 +1,80 @@
+"""Host-side checks on the CPU: the C-ABI library loads and exports every function
+declared in include/*.h (no GPU call is made); FFT selection matches the
+reference's documented choices; the workload generators match their definition."""
+import os
+import re
+
+import pytest
+
+import gpuowl_amd
+from gpuowl_amd import workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(owl_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gpuowl_amd.load()
+    names = declared_functions()
+    assert len(names) >= 40
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+
+
+def test_no_fallback_when_library_missing(monkeypatch, tmp_path):
+    from gpuowl_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(_lib.OwlError):
+        _lib.load()
+
+
+@pytest.mark.parametrize("E,spec,N", [
+    (4_000_037, "256:2:256:101", 262144),        # config A
+    (139_000_003, "1K:15:256:101", 7864320),      # config B, maxExp ~142.56M
+    (1_029_689, "256:2:256:101", 262144),
+    (56_430_277, "1K:6:256:101", 3145728),
+])
+def test_best_fit_matches_reference_choice(E, spec, N):
+    s, n, mx = gpuowl_amd.best_fit(E)
+    assert (s, n) == (spec, N)
+    assert mx >= E
+
+
+def test_best_fit_config_b_max_exp():
+    s, n, mx = gpuowl_amd.best_fit(139_000_003)
+    assert abs(mx - 142_560_000) < 10_000
+
+
+def test_best_fit_config_d_distribution():
+    ps = workloads.config_d()
+    assert ps[:5] == [19_917_551, 11_330_491, 2_287_861, 3_503_053, 19_297_693]
+    assert min(ps) == 1_029_689 and max(ps) == 56_430_277
+    Ns = [gpuowl_amd.best_fit(p)[1] for p in ps]
+    assert Ns.count(262144) == 83 and Ns.count(393216) == 22
+    assert sum(Ns) == 155_713_536
+    assert sum(1 for p, n in zip(ps, Ns) if p / n < 12) == 58
+
+
+def test_spec_parsing_and_errors():
+    assert gpuowl_amd.best_fit(1000, "1K:3:256")[1] == 1572864
+    assert gpuowl_amd.best_fit(1000, "1.5M")[1] == 1572864
+    assert gpuowl_amd.best_fit(1000, "256:2:256:202:1")[0] == "256:2:256:202:1"
+    for bad in ("1K:3", "1K:3:256:999", "1K:3:256:101:7", "0:1:1"):
+        with pytest.raises(gpuowl_amd.OwlError):
+            gpuowl_amd.best_fit(1000, bad)
+
+
+def test_workloads():
+    assert workloads.config_a() == 4_000_037
+    assert workloads.config_b() == 139_000_003
+    c = workloads.config_c()
+    assert len(c) == 64 and c[0] == 20_000_003 and c[-1] == 20_000_879
