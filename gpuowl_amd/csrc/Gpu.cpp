@@ -145,12 +145,12 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
   for (int i = 0; i < 3; ++i) { J.freq[i] = alloc<double2>(ND); }
   for (int i = 0; i < 4; ++i) { J.words[i] = alloc<int2>(ND); }
   for (int i = 0; i < 2; ++i) { J.carry[i] = alloc<int64_t>(size_t(nb) * W); }
-  J.ready = alloc<uint32_t>(nb + 1);
+  J.stair = alloc<uint64_t>(size_t(nb) * W);
   J.roe = alloc<uint32_t>(OWL_ROE_SIZE);
   J.flags = alloc<uint32_t>(8);
   J.small = alloc<int32_t>(64);
   J.sum = alloc<uint64_t>(1);
-  owlCheck(owl_hip_fill32(queue_, J.ready, 0, nb + 1), "fill");
+  owlCheck(owl_hip_fill32(queue_, J.stair, 0, 2 * nb * W), "fill");
   owlCheck(owl_hip_fill32(queue_, J.roe, 0, OWL_ROE_SIZE), "fill");
   owlCheck(owl_hip_fill32(queue_, J.flags, 0, 8), "fill");
   for (int i = 0; i < 4; ++i) { owlCheck(owl_hip_fill32(queue_, J.words[i], 0, 2 * ND), "fill"); }
@@ -191,14 +191,17 @@ u32 Gpu::nextRoeSlot() {
 void Gpu::rowFused(bool ll, u32 roeSlot) {
   const JobDesc* j = jobs_.get();
   int mode = ROW_FUSED, fi = 0, wi = 0, opts = ll ? 1 : 0;
-  kRow(queue_, nb + 1, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot});
+  // consecutive fused launches must use different tags (also across graph replays)
+  stairTag_ = stairTag_ % 0xFFFE + 1;
+  u32 tag = stairTag_;
+  kRow(queue_, nb + 1, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot, &tag});
 }
 
 void Gpu::rowLeadInTo(Buf in, int freqIdx) {
   const JobDesc* j = jobs_.get();
   int mode = ROW_LEADIN, fi = freqIdx, wi = in, opts = 0;
-  u32 slot = OWL_ROE_SIZE;
-  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &slot});
+  u32 slot = OWL_ROE_SIZE, tag = 0;
+  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &slot, &tag});
 }
 
 void Gpu::rowLeadIn(Buf in) { rowLeadInTo(in, 0); }
@@ -206,7 +209,8 @@ void Gpu::rowLeadIn(Buf in) { rowLeadInTo(in, 0); }
 void Gpu::rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot) {
   const JobDesc* j = jobs_.get();
   int mode = ROW_CARRYOUT, fi = 0, wi = out, opts = (ll ? 1 : 0) | (mul3 ? 2 : 0);
-  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot});
+  u32 tag = 0;
+  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot, &tag});
 }
 
 void Gpu::carryAll(Buf b) {

@@ -123,6 +123,7 @@ private:
   Buffer<JobDesc> jobs_;
   std::vector<std::unique_ptr<Buffer<char>>> owned_;
   u32 roePos = 0;
+  u32 stairTag_ = 0;
   Kernel kRow, kCol, kCarryB, kCarryFix, kWordsIn, kWordsOut, kResidue, kIsEqual, kSum64, kCopy;
 
   template<typename T> T* alloc(size_t n);

@@ -60,8 +60,7 @@ struct JobDesc {
   double2* freq[3];       // [0] state, [1] scratch, [2] second-operand spectrum
   int2* words[4];         // [0] data, [1] aux, [2] check, [3] base: integer words (line-block layout)
   int64_t* carry[2];      // carry shuttle (nb*W) and carry-propagation ping-pong
-  uint32_t* ready;        // nb + 1 stairway flags: [b] block b's carries are published,
-                          // [nb] block 0 has been read (it is rewritten in place by group nb)
+  uint64_t* stair;        // nb*W tagged carries of the fused row pass (tag << 48 | 48-bit carry)
   uint32_t* roe;          // OWL_ROE_SIZE max-round-off slots (float bits)
   uint32_t* flags;        // [0] carry leftover, [1] compare mismatch
   int32_t* small;         // 64 ints for the residue read

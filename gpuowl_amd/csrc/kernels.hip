@@ -23,6 +23,50 @@ template<int R> struct Roots;
 
 typedef double2 T2;
 
+// Pointers read from a JobDesc are generic; these helpers access them through the
+// global address space so that the compiler emits global_load/store (not flat_*).
+#define GLOBAL __attribute__((address_space(1)))
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef int v2i __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ T2 gld(const T2* p) {
+  v2d v = *(const GLOBAL v2d*)p;
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(T2* p, T2 x) {
+  v2d v = {x.x, x.y};
+  *(GLOBAL v2d*)p = v;
+}
+__device__ __forceinline__ int2 gld(const int2* p) {
+  v2i v = *(const GLOBAL v2i*)p;
+  return make_int2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(int2* p, int2 x) {
+  v2i v = {x.x, x.y};
+  *(GLOBAL v2i*)p = v;
+}
+__device__ __forceinline__ int64_t gld(const int64_t* p) { return *(const GLOBAL int64_t*)p; }
+__device__ __forceinline__ void gst(int64_t* p, int64_t x) { *(GLOBAL int64_t*)p = x; }
+__device__ __forceinline__ LineW gld(const LineW* p) {
+  const GLOBAL LineW* g = (const GLOBAL LineW*)p;
+  LineW r;
+  r.w[0] = g->w[0];
+  r.w[1] = g->w[1];
+  r.iw[0] = g->iw[0];
+  r.iw[1] = g->iw[1];
+  r.e[0] = g->e[0];
+  r.e[1] = g->e[1];
+  return r;
+}
+__device__ __forceinline__ ColW gld(const ColW* p) {
+  const GLOBAL ColW* g = (const GLOBAL ColW*)p;
+  ColW r;
+  r.w = g->w;
+  r.iw = g->iw;
+  r.e = g->e;
+  r.pad = 0;
+  return r;
+}
+
 // ---------------------------------------------------------------- complex helpers
 __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ T2 csub(T2 a, T2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -181,7 +225,7 @@ __device__ __noinline__ void fftStage(T2* lds, int L, int nfft, int Ns, const T2
         int st = p * stride;
 #pragma unroll
         for (int k = 1; k < R; ++k) {
-          T2 w = tw[st * k];
+          T2 w = gld(tw + st * k);
           v[m][k] = INV ? cmulc(v[m][k], w) : cmul(v[m][k], w);
         }
       }
@@ -286,12 +330,35 @@ __device__ __forceinline__ void roeReport(const JobDesc& J, uint32_t slot, float
 // Grid (nb + 1, batch) for ROW_FUSED, (nb, batch) otherwise; LDS R*W*16 bytes.
 // opts: bit0 = LL (-2 as carry-in of word 0), bit1 = MUL3 (x3 before the carry).
 // ROW_FUSED: freq[fi] -> freq[fi] in place (group nb redoes block 0 with the
-//   wrap-around carry, as the reference carryFused does with group H; ready[]
-//   has nb + 1 flags).
+//   wrap-around carry, as the reference carryFused does with group H). tag must
+//   differ from the previous fused launch's (1..0xFFFE).
 // ROW_LEADIN: words[wi] -> freq[fi].
 // ROW_CARRYOUT: freq[fi] -> words[wi] + carry[0] (block-normalized words).
+// Stairway carries are tagged: bits 48..63 hold the launch tag, bits 0..47 the
+// signed carry. A consumer polls its column's carry until the tag is current, so
+// no flags, fences or barriers are needed between the groups.
+__device__ __forceinline__ uint64_t packCarry(int64_t c, uint32_t tag) {
+  return ((uint64_t)c & 0x0000FFFFFFFFFFFFull) | ((uint64_t)tag << 48);
+}
+__device__ __forceinline__ int64_t unpackCarry(uint64_t v) { return (int64_t)(v << 16) >> 16; }
+
+__device__ __forceinline__ uint64_t pollCarry(const JobDesc& J, uint64_t* p, uint32_t tag) {
+  uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t n = 0;
+  while ((uint32_t)(v >> 48) != tag) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++n >= OWL_SPIN_LIMIT) {
+      atomicOr(J.flags + 2, 1u);  // hand-off timed out: flagged to the host, kernel still exits
+      break;
+    }
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return v;
+}
+
 template<int T>
-__global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts, uint32_t roeSlot) {
+__global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts,
+                                           uint32_t roeSlot, uint32_t tag) {
   extern __shared__ T2 lds[];
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, nb = J.nb;
@@ -306,24 +373,24 @@ __global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int
   if (mode == ROW_LEADIN) {
     const int2* src = J.words[wi] + (size_t)lb * LW;
     for (int c = tid; c < W; c += T) {
-      for (int r = 0; r < R; ++r) { ws[2 * (r * W + c)] = src[c * R + r]; }
+      for (int r = 0; r < R; ++r) { ws[2 * (r * W + c)] = gld(src + c * R + r); }
     }
   } else {
     for (int e = tid; e < LW; e += T) {
       int s = e / R;
       int r = e - s * R;
-      lds[r * W + slotFreq(s, W)] = blk[e];
+      lds[r * W + slotFreq(s, W)] = gld(blk + e);
     }
     __syncthreads();
     fftLds<true, T>(lds, W, R, J.rowPlan, J.nRow, J.twRow, tid);
 
     float roe = 0;
     for (int c = tid; c < W; c += T) {
-      const ColW cw = J.colW[c];
+      const ColW cw = gld(J.colW + c);
       int64_t s = 0;
       for (int r = 0; r < R; ++r) {
         const int l = lb * R + r;
-        const LineW lw = J.lineW[l];
+        const LineW lw = gld(J.lineW + l);
         T2 z = lds[r * W + c];
         int32_t d[2];
 #pragma unroll
@@ -340,9 +407,9 @@ __global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int
         ws[2 * (r * W + c)] = make_int2(d[0], d[1]);
       }
       if (mode == ROW_CARRYOUT) {
-        J.carry[0][(size_t)lb * W + c] = s;
+        gst(J.carry[0] + (size_t)lb * W + c, s);
       } else if (g < nb) {
-        __hip_atomic_store(J.carry[0] + (size_t)g * W + c, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(J.stair + (size_t)g * W + c, packCarry(s, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     roeReport(J, roeSlot, roe);
@@ -350,52 +417,32 @@ __global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int
     if (mode == ROW_CARRYOUT) {
       int2* dst = J.words[wi] + (size_t)lb * LW;
       for (int c = tid; c < W; c += T) {
-        for (int r = 0; r < R; ++r) { dst[c * R + r] = ws[2 * (r * W + c)]; }
+        for (int r = 0; r < R; ++r) { gst(dst + c * R + r, ws[2 * (r * W + c)]); }
       }
       return;
     }
 
-    // Stairway: publish this block's carries, then take the previous block's.
-    // Group 0 also raises ready[nb] once it is done with block 0 (its carries
-    // depend on every value it read), because group nb rewrites block 0 in place.
-    if (g < nb) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __syncthreads();
-      if (tid == 0) {
-        __hip_atomic_store(J.ready + g, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (g == 0) { __hip_atomic_store(J.ready + nb, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
-      }
-    }
+    // Stairway: block g's carries were published above; take the previous block's.
+    // Group 0 does not write: group nb redoes block 0 with the wrap-around carry.
     if (g == 0) { return; }
     const int pred = g - 1;
-    if (tid == 0) {
-      for (int w = 0; w < (g == nb ? 2 : 1); ++w) {
-        uint32_t* flag = J.ready + (w ? nb : pred);
-        uint32_t n = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++n >= OWL_SPIN_LIMIT) {
-            atomicOr(J.flags + 2, 1u);  // hand-off timed out: flagged to the host, kernel still exits
-            break;
-          }
-        }
-        __hip_atomic_store(flag, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     for (int c = tid; c < W; c += T) {
       // block 0 takes the carry of the last block of the previous column (word order
       // wraps from column W-1 to column 0 because 2^E == 1 mod 2^E - 1)
       int srcCol = (g == nb) ? (c == 0 ? W - 1 : c - 1) : c;
-      int64_t carry = __hip_atomic_load(J.carry[0] + (size_t)pred * W + srcCol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t carry = unpackCarry(pollCarry(J, J.stair + (size_t)pred * W + srcCol, tag));
+      if (g == nb) {
+        // block 0 is rewritten in place: wait until group 0 has published a carry,
+        // which it computes only after every one of its loads of block 0 has returned
+        pollCarry(J, J.stair + c, tag);
+      }
       // Ripple the incoming carry through this block's 2R words; what survives the
       // block (rare: the host picks R so the carry dies well inside it) stays in the
       // last word. Leaving it all in the second word, as a one-step hand-off does,
       // makes the iteration diverge below about 12 bits per word.
-      const ColW cw = J.colW[c];
+      const ColW cw = gld(J.colW + c);
       for (int r = 0; r < R && carry; ++r) {
-        const LineW lw = J.lineW[lb * R + r];
+        const LineW lw = gld(J.lineW + lb * R + r);
         int2 d = ws[2 * (r * W + c)];
         WordW w0 = wordWeight(J, lw, cw, 0);
         int64_t t = (int64_t)d.x + carry;
@@ -419,9 +466,9 @@ __global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int
 
   // Weight the digits and run the forward row FFT.
   for (int c = tid; c < W; c += T) {
-    const ColW cw = J.colW[c];
+    const ColW cw = gld(J.colW + c);
     for (int r = 0; r < R; ++r) {
-      const LineW lw = J.lineW[lb * R + r];
+      const LineW lw = gld(J.lineW + lb * R + r);
       int2 d = ws[2 * (r * W + c)];
       WordW w0 = wordWeight(J, lw, cw, 0);
       WordW w1 = wordWeight(J, lw, cw, 1);
@@ -433,13 +480,13 @@ __global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int
   for (int e = tid; e < LW; e += T) {
     int s = e / R;
     int r = e - s * R;
-    blk[e] = lds[r * W + slotFreq(s, W)];
+    gst(blk + e, lds[r * W + slotFreq(s, W)]);
   }
 }
 
 // ---------------------------------------------------------------- column pass
 __device__ __forceinline__ T2 midTw(const JobDesc& J, int a, int l) {
-  return cmul(J.twMidA[a * 64 + (l & 63)], J.twMidB[a * J.nHi + (l >> 6)]);
+  return cmul(gld(J.twMidA + a * 64 + (l & 63)), gld(J.twMidB + a * J.nHi + (l >> 6)));
 }
 
 // Real-signal pairing for the squaring (see the derivation in the host docs):
@@ -482,7 +529,7 @@ __global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int
     int wi = e - lb * per;
     int h = wi / R;
     int l = lb * R + wi - h * R;
-    T2 v = io[(size_t)lb * W * R + colBase + wi];
+    T2 v = gld(io + (size_t)lb * W * R + colBase + wi);
     lds[h * H + l] = cmul(v, midTw(J, h ? a1 : a0, l));
   }
   __syncthreads();
@@ -494,7 +541,7 @@ __global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int
       int wi = e - lb * per;
       int h = wi / R;
       int b = lb * R + wi - h * R;
-      io[(size_t)lb * W * R + colBase + wi] = lds[h * H + b];
+      gst(io + (size_t)lb * W * R + colBase + wi, lds[h * H + b]);
     }
     return;
   }
@@ -503,14 +550,15 @@ __global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int
   // position of spectrum bin b of half h in a COL_FWD output buffer
   auto spos = [&](int h, int b) -> size_t { return (size_t)(b / R) * W * R + colBase + h * R + (b % R); };
 
+  const T2* twCol = J.twCol;
   if (q) {
-    const T2 wa = J.twPair[a0];
+    const T2 wa = gld(J.twPair + a0);
     for (int b = tid; b < H; b += T) {
       int bp = H - 1 - b;
       T2 X = lds[b], Xp = lds[H + bp];
-      T2 w = cmul(wa, J.twCol[b]);
+      T2 w = cmul(wa, gld(twCol + b));
       if (other) {
-        pairMul(X, Xp, other[spos(0, b)], other[spos(1, bp)], w);
+        pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(1, bp)), w);
       } else {
         pairSq(X, Xp, w);
       }
@@ -520,14 +568,14 @@ __global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int
   } else {
     // column 0 pairs b with H-b (b = 0 and b = H/2 pair with themselves);
     // column W/2 pairs b with H-1-b.
-    const T2 wa = J.twPair[W / 2];
+    const T2 wa = gld(J.twPair + W / 2);
     for (int i = tid; i <= H; i += T) {
       if (i <= H / 2) {
         int b = i, bp = (H - i) % H;
         T2 X = lds[b], Xp = lds[bp];
-        T2 w = J.twCol[b];
+        T2 w = gld(twCol + b);
         if (other) {
-          pairMul(X, Xp, other[spos(0, b)], other[spos(0, bp)], w);
+          pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(0, bp)), w);
         } else {
           pairSq(X, Xp, w);
         }
@@ -536,9 +584,9 @@ __global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int
       } else {
         int b = i - H / 2 - 1, bp = H - 1 - b;
         T2 X = lds[H + b], Xp = lds[H + bp];
-        T2 w = cmul(wa, J.twCol[b]);
+        T2 w = cmul(wa, gld(twCol + b));
         if (other) {
-          pairMul(X, Xp, other[spos(1, b)], other[spos(1, bp)], w);
+          pairMul(X, Xp, gld(other + spos(1, b)), gld(other + spos(1, bp)), w);
         } else {
           pairSq(X, Xp, w);
         }
@@ -548,14 +596,14 @@ __global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int
     }
   }
   __syncthreads();
-  fftLds<true, T>(lds, H, 2, J.colPlan, J.nCol, J.twCol, tid);
+  fftLds<true, T>(lds, H, 2, J.colPlan, J.nCol, twCol, tid);
 
   for (int e = tid; e < 2 * H; e += T) {
     int lb = e / per;
     int wi = e - lb * per;
     int h = wi / R;
     int l = lb * R + wi - h * R;
-    io[(size_t)lb * W * R + colBase + wi] = cmulc(lds[h * H + l], midTw(J, h ? a1 : a0, l));
+    gst(io + (size_t)lb * W * R + colBase + wi, cmulc(lds[h * H + l], midTw(J, h ? a1 : a0, l)));
   }
 }
 
