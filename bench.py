@@ -52,6 +52,92 @@ def cpu_baseline(E, iters):
             "secs": rec["secs"]}
 
 
+def engine_key(info):
+    return (info["spec"], info["R"], info["long_carry"], info["row"], info["col"])
+
+
+class PlainLoop:
+    """Steady squaring time of one exponent alone (the plain one-exponent-at-a-time loop),
+    measured once per engine geometry."""
+
+    def __init__(self, owl, iters=200, warmup=20):
+        self.owl, self.iters, self.warmup, self.us = owl, iters, warmup, {}
+
+    def us_per_iter(self, E, spec=""):
+        g = self.owl.Gpu(E, spec)
+        key = (g.spec, g.R, g.long_carry, g.row_kernel, g.col_kernel)
+        if key not in self.us:
+            self.us[key] = g.time_squares(self.iters, self.warmup, use_graph=False)
+        g.close()
+        return self.us[key]
+
+
+def batch_leg(owl, exps, spec, iters, warmup):
+    """Batched squarings (graph-replayed) vs the plain loop: the same exponents one at a time."""
+    b = owl.Batch(exps, spec)
+    us_batch = b.time_squares(iters, warmup, use_graph=True)
+    words = b.total_words
+    shapes = {}
+    for i in range(b.n):
+        shapes.setdefault(engine_key(b.job_info(i)), []).append(exps[i])
+    groups = b.n_groups
+    b.close()
+    plain = PlainLoop(owl, iters, warmup)
+    plain_us = 0.0
+    for key, es in shapes.items():
+        plain_us += plain.us_per_iter(es[0], key[0]) * len(es)
+    rate = words / (us_batch * 1e-6)
+    plain = words / (plain_us * 1e-6)
+    return {"jobs": len(exps), "groups": groups, "total_words": words, "us_per_batch_iter": round(us_batch, 2),
+            "squarings_words_per_s": round(rate), "plain_loop_squarings_words_per_s": round(plain),
+            "speedup_vs_plain": round(plain_us / us_batch, 3),
+            "hbm_frac_32N_model": round(32 * rate / (HBM_PEAK_GBS * 1e9), 4)}
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    i = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def serve_leg(owl, workloads, count, lam, slice_iters, full_below, block=1000, check=50_000):
+    """Poisson arrivals (splitmix64 seed 7) of PRP-3 / LL requests, exponents log-uniform primes
+    in [2e6, 1.4e8]: below full_below a request runs to completion (E or E - 2 iterations),
+    otherwise one slice of slice_iters. Requests are submitted at their arrival time and the
+    server steps rounds of `block` iterations; latency = completion - arrival."""
+    reqs = workloads.config_e(count, lam)
+    length = [(E - 2 if kind == "ll" else E) if E < full_below else slice_iters for _, E, kind in reqs]
+    plain = PlainLoop(owl)  # measured before the served run starts
+    plain_s = sum(n * plain.us_per_iter(E) * 1e-6 for n, (_, E, _) in zip(length, reqs))
+    s = owl.Server(block_size=block, check_step=check)
+    t0 = s.now()
+    arrive, res, i = {}, [], 0
+    while len(res) < count:
+        now = s.now() - t0
+        while i < count and reqs[i][0] <= now:
+            t, E, kind = reqs[i]
+            arrive[s.submit(E, kind, length[i])] = t
+            i += 1
+        inf = s.info()
+        if not inf["active"] and not inf["waiting"] and i < count:
+            time.sleep(max(0.0, reqs[i][0] - (s.now() - t0)))
+            continue
+        res += s.step()
+    wall = s.now() - t0
+    inf = s.info()
+    s.close()
+    lat = [r["tDone"] - t0 - arrive[r["id"]] for r in res]
+    bad = [r for r in res if r["status"] != 0 or r["nErrors"] or r["maxRoe"] >= 0.4]
+    return {"requests": count, "lambda_per_s": lam, "slice_iters": slice_iters, "full_below": full_below,
+            "block": block, "check_step": check, "wall_s": round(wall, 2), "rounds": inf["rounds"],
+            "latency_p50_s": round(percentile(lat, 50), 3), "latency_p99_s": round(percentile(lat, 99), 3),
+            "iter_per_s": round(inf["iterations"] / wall), "squarings_words_per_s": round(inf["wordIterations"] / wall),
+            "plain_loop_equiv_s": round(plain_s, 2), "speedup_vs_plain": round(plain_s / wall, 3),
+            "max_roe": round(max(r["maxRoe"] for r in res), 4), "failed_or_unclean": len(bad)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +145,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched legs (configs C and D)")
+    ap.add_argument("--batch-iters", type=int, default=100)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving sample")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-lambda", type=float, default=4.0)
+    ap.add_argument("--serve-slice", type=int, default=20_000)
+    ap.add_argument("--serve-full-below", type=int, default=0,
+                    help="requests below this exponent run to completion (the full serving config uses 10,000,000)")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the profiled workload: K steady iterations and one state copy, no JSON line")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (exponents shard across GPUs "
@@ -71,6 +167,13 @@ def main():
     E = workloads.config_b()
     g = owl.Gpu(E)
     N = g.N
+
+    if a.profile:
+        us = g.time_squares(a.steps, a.warmup, use_graph=False)
+        us_copy = g.time_copy(1)
+        g.close()
+        print(f"profile workload: {a.warmup}+{a.steps} iterations, {us:.1f} us/iter; copy {us_copy:.1f} us")
+        return
 
     # parity: res64 against the GMP golden, before timing
     parity = None
@@ -88,6 +191,7 @@ def main():
     us_graph = g.time_squares(a.steps, a.warmup, use_graph=True)
     us_prp = g.time_prp(iters=max(1000, a.steps // 1000 * 1000), warmup=a.warmup, block_size=1000)
     us_col, us_row = g.time_kernels(200)
+    us_copy = g.time_copy(50)
     roe = g.read_roe()
 
     alg_bytes = 16 * N  # per launch: the FP64 state (8 B per word) read once and written once
@@ -98,8 +202,20 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_col (column pass: pairing + square)" if dom == "col" else "k_row (row pass: fused carry)",
                 "us_per_launch": {"col": round(us_col, 2), "row": round(us_row, 2)},
-                "alg_bytes_per_launch": alg_bytes}
+                "alg_bytes_per_launch": alg_bytes,
+                "copy_gbs": round(alg_bytes / (us_copy * 1e-6) / 1e9, 1)}
     g.close()
+
+    batched = None
+    if not a.no_batch:
+        batched = {
+            "C_uniform_64x1K:3:256": batch_leg(owl, workloads.config_c(), "1K:3:256", a.batch_iters, 10),
+            "D_uneven_200": batch_leg(owl, workloads.config_d(), "", a.batch_iters, 10),
+        }
+
+    served = None
+    if not a.no_serve:
+        served = serve_leg(owl, workloads, a.serve_requests, a.serve_lambda, a.serve_slice, a.serve_full_below)
 
     cpu = None
     if not a.no_cpu:
@@ -112,7 +228,8 @@ def main():
                        "parallelism": "single exponent, one GPU"},
             "roofline": roofline, "cpu_baseline": cpu,
             "extra": {"us_per_iter": round(us, 2), "us_per_iter_graph": round(us_graph, 2),
-                      "us_per_iter_prp_with_gerbicz": round(us_prp, 2), "roe": roe, "parity": parity}}
+                      "us_per_iter_prp_with_gerbicz": round(us_prp, 2), "roe": roe, "parity": parity,
+                      "batched": batched, "served": served}}
     print(json.dumps(line))
 
 

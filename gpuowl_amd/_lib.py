@@ -16,7 +16,8 @@ class OwlError(RuntimeError):
 class GpuInfo(C.Structure):
     _fields_ = [("E", C.c_uint32), ("N", C.c_uint32), ("W", C.c_uint32), ("H", C.c_uint32), ("R", C.c_uint32),
                 ("nb", C.c_uint32), ("rowThreads", C.c_uint32), ("colThreads", C.c_uint32),
-                ("maxExp", C.c_uint32), ("longCarry", C.c_int), ("spec", C.c_char * 64)]
+                ("maxExp", C.c_uint32), ("longCarry", C.c_int), ("spec", C.c_char * 64),
+                ("rowKernel", C.c_char * 32), ("colKernel", C.c_char * 32)]
 
 
 class PrpResult(C.Structure):
@@ -26,6 +27,19 @@ class PrpResult(C.Structure):
 
 class LLResult(C.Structure):
     _fields_ = [("res64", C.c_uint64), ("k", C.c_uint32), ("isPrime", C.c_int), ("secsPerIt", C.c_double)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_uint32), ("E", C.c_uint32), ("iters", C.c_uint32), ("ll", C.c_int),
+                ("res64", C.c_uint64), ("isPrime", C.c_int), ("nErrors", C.c_uint32), ("nChecks", C.c_uint32),
+                ("maxRoe", C.c_double), ("tSubmit", C.c_double), ("tAdmit", C.c_double), ("tDone", C.c_double),
+                ("status", C.c_int)]
+
+
+class ServeInfo(C.Structure):
+    _fields_ = [("waiting", C.c_uint32), ("active", C.c_uint32), ("groups", C.c_uint32), ("done", C.c_uint32),
+                ("rounds", C.c_uint32), ("bytesInUse", C.c_uint64), ("budget", C.c_uint64),
+                ("iterations", C.c_uint64), ("wordIterations", C.c_uint64)]
 
 
 u32, i32, u64, dbl, vp, cp = C.c_uint32, C.c_int, C.c_uint64, C.c_double, C.c_void_p, C.c_char_p
@@ -72,12 +86,29 @@ SIGNATURES = {
     "owl_gpu_copy": [vp, i32, i32],
     "owl_gpu_read_roe": [vp, P(u32), P(dbl), P(dbl), P(dbl)],
     "owl_gpu_prp": [vp, u32, u32, u32, P(u64), u32, u32, P(PrpResult)],
+    "owl_gpu_arm_fault": [vp, u32],
     "owl_gpu_ll": [vp, u32, P(u64), u32, u32, P(LLResult)],
     "owl_gpu_time_prp": [vp, u32, u32, u32, P(dbl)],
     "owl_gpu_time_squares": [vp, u32, u32, i32, P(dbl)],
     "owl_gpu_time_kernels": [vp, u32, P(dbl), P(dbl)],
     "owl_gpu_freq_io": [vp, i32, P(dbl), u32, i32],
+    "owl_gpu_time_copy": [vp, u32, P(dbl)],
     "owl_gpu_pass": [vp, i32, i32],
+    "owl_batch_create": [P(u32), u32, cp, i32, P(vp)],
+    "owl_batch_destroy": [vp],
+    "owl_batch_info": [vp, P(u32), P(u32), P(u64)],
+    "owl_batch_job_info": [vp, u32, P(GpuInfo)],
+    "owl_batch_prp": [vp, u32, u32, u32, P(u64), u32, u32, P(PrpResult)],
+    "owl_batch_arm_fault": [vp, u32, u32],
+    "owl_batch_ll": [vp, u32, P(u64), u32, u32, P(LLResult)],
+    "owl_batch_time_squares": [vp, u32, u32, i32, P(dbl)],
+    "owl_server_create": [u64, u32, u32, u32, cp, i32, P(vp)],
+    "owl_server_destroy": [vp],
+    "owl_server_submit": [vp, u32, i32, u32, P(u32)],
+    "owl_server_step": [vp, P(u32)],
+    "owl_server_take": [vp, P(ServeResult), u32, P(u32)],
+    "owl_server_info": [vp, P(ServeInfo)],
+    "owl_server_now": [vp, P(dbl)],
 }
 
 _lib = None

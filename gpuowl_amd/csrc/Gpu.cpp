@@ -2,6 +2,7 @@
 #include "Gpu.h"
 
 #include "Trig.h"
+#include "TrigBufCache.h"
 
 #include <algorithm>
 #include <chrono>
@@ -15,8 +16,6 @@ u32 pow2AtLeast(u32 x, u32 lo, u32 hi) {
   while (t < x && t < hi) { t *= 2; }
   return t;
 }
-
-double2 d2(std::pair<double, double> p) { return double2{p.first, p.second}; }
 
 }  // namespace
 
@@ -74,6 +73,16 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
   nb = H / R;
   useLongCarry = useLongCarry || !fusedStable(R);
 
+  // shape-specialized passes when compiled for this shape (and R fits their lane mapping)
+  auto haveKernel = [](const std::string& name) {
+    owl_kernel k = nullptr;
+    return owl_hip_kernel_get(name.c_str(), &k) == 0;
+  };
+  rowKernel = "rowx" + std::to_string(W) + "r" + std::to_string(R);
+  colKernel = "colx" + std::to_string(H);
+  bool rowFixed = !opt.generic && haveKernel(rowKernel);
+  bool colFixed = !opt.generic && haveKernel(colKernel) && (H / 16) % R == 0;
+
   rowPlan = fftPlan(W);
   colPlan = fftPlan(H);
   if (rowPlan.size() > OWL_MAX_STAGES || colPlan.size() > OWL_MAX_STAGES || rowPlan.empty() || colPlan.empty()) {
@@ -86,20 +95,31 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
   if (!planFits(rowPlan, W, R, rowT, 16) || !planFits(colPlan, H, 2, colT, 16)) {
     throw std::invalid_argument("FFT plan does not fit the register budget for " + fft.spec());
   }
+  rowLds = R * W * 16;
+  colLds = 2 * H * 16;
+  if (rowFixed) {
+    owl_kernel k = nullptr;
+    owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
+    owlCheck(owl_hip_kernel_max_threads(k, &rowT), "kernel");
+    rowLds = R * (W + W / 16) * 16;
+  } else {
+    rowKernel = "row" + std::to_string(rowT);
+  }
+  if (colFixed) {
+    owl_kernel k = nullptr;
+    owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
+    owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
+    colLds = 2 * (H + H / 16) * 16;
+  } else {
+    colKernel = "col" + std::to_string(colT);
+  }
 
   owlCheck(owl_hip_queue_create(&queue_), "queue");
 
-  // ---- host tables (long double, rounded once)
+  // ---- tables: twiddles shared per shape (TrigBufCache), weights per exponent (long double, rounded once)
   const u32 step = stepOf(N, E);
   const u32 nHi = (H + 63) / 64;
-  std::vector<double2> twRow(W), twCol(H), twMidA(size_t(W) * 64), twMidB(size_t(W) * nHi), twPair(W);
-  for (u32 m = 0; m < W; ++m) { twRow[m] = d2(root1(W, m)); }
-  for (u32 m = 0; m < H; ++m) { twCol[m] = d2(root1(H, m)); }
-  for (u32 a = 0; a < W; ++a) {
-    twPair[a] = d2(root1(ND, a));
-    for (u32 lo = 0; lo < 64; ++lo) { twMidA[size_t(a) * 64 + lo] = d2(root1(ND, u64(a) * lo)); }
-    for (u32 hi = 0; hi < nHi; ++hi) { twMidB[size_t(a) * nHi + hi] = d2(root1(ND, u64(a) * 64 * hi % ND)); }
-  }
+  trig_ = TrigBufCache::get(W, H, queue_);
   std::vector<LineW> lineW(H);
   for (u32 l = 0; l < H; ++l) {
     for (u32 rep = 0; rep < 2; ++rep) {
@@ -129,17 +149,20 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
   J.bitsLo = E / N;
   J.rE = E % N;
   J.nHi = nHi;
+  J.lgR = 0;
+  while ((1u << J.lgR) < R) { ++J.lgR; }
   J.nRow = u32(rowPlan.size());
   J.nCol = u32(colPlan.size());
   for (u32 i = 0; i < OWL_MAX_STAGES; ++i) {
     J.rowPlan[i] = i < rowPlan.size() ? rowPlan[i] : 0;
     J.colPlan[i] = i < colPlan.size() ? colPlan[i] : 0;
   }
-  J.twRow = upload(twRow);
-  J.twCol = upload(twCol);
-  J.twMidA = upload(twMidA);
-  J.twMidB = upload(twMidB);
-  J.twPair = upload(twPair);
+  J.twRow = trig_->row.get();
+  J.twCol = trig_->col.get();
+  J.twMidA = trig_->midA.get();
+  J.twMidB = trig_->midB.get();
+  J.twPair = trig_->pair.get();
+  J.twMidS = trig_->midS.get();
   J.lineW = upload(lineW);
   J.colW = upload(colW);
   for (int i = 0; i < 3; ++i) { J.freq[i] = alloc<double2>(ND); }
@@ -157,8 +180,8 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
   jobs_ = Buffer<JobDesc>(1);
   jobs_.write(queue_, {J});
 
-  kRow = Kernel("row" + std::to_string(rowT));
-  kCol = Kernel("col" + std::to_string(colT));
+  kRow = Kernel(rowKernel);
+  kCol = Kernel(colKernel);
   kCarryB = Kernel("carryB");
   kCarryFix = Kernel("carryFix");
   kWordsIn = Kernel("wordsIn");
@@ -174,6 +197,7 @@ Gpu::~Gpu() {
   if (queue_) {
     owl_hip_finish(queue_);
     owned_.clear();
+    trig_.reset();
     jobs_ = Buffer<JobDesc>();
     owl_hip_queue_destroy(queue_);
   }
@@ -187,86 +211,159 @@ u32 Gpu::nextRoeSlot() {
   return s;
 }
 
-// ---------------------------------------------------------------- launches
-void Gpu::rowFused(bool ll, u32 roeSlot) {
-  const JobDesc* j = jobs_.get();
-  int mode = ROW_FUSED, fi = 0, wi = 0, opts = ll ? 1 : 0;
-  // consecutive fused launches must use different tags (also across graph replays)
-  stairTag_ = stairTag_ % 0xFFFE + 1;
-  u32 tag = stairTag_;
-  kRow(queue_, nb + 1, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot, &tag});
+u32 Gpu::nextStairTag() {
+  do {
+    stairTag_ = stairTag_ % 0xFFFE + 1;
+  } while (stairTag_ == lastTag_ || (stairTag_ >= resLo_ && stairTag_ <= resHi_));
+  return lastTag_ = stairTag_;
 }
 
-void Gpu::rowLeadInTo(Buf in, int freqIdx) {
+owl_graph Gpu::captureFused(u32 G, bool ll, u32 roeSlot, Launch L, u32* lastTagOut) {
+  if (G < 2) { throw std::invalid_argument("captureFused: at least 2 squarings"); }
+  const u32 before = lastTag_;
+  resLo_ = 1, resHi_ = 0;
+  if (stairTag_ + G + 2 > 0xFFFE) { stairTag_ = 0; }  // keep the captured range unwrapped
+  owl_graph g = nullptr;
+  owlCheck(owl_hip_graph_begin(L.q), "graph begin");
+  u32 first = 0;
+  for (u32 i = 0; i < G; ++i) {
+    colSquare(L);
+    rowFused(ll, roeSlot, L);
+    if (!i) { first = lastTag_; }
+  }
+  owlCheck(owl_hip_graph_end(L.q, &g), "graph end");
+  // the graph's tags are contiguous (up to one step over `before`): no later direct tag
+  // may fall in [first, last]
+  u32 last = lastTag_;
+  resLo_ = first, resHi_ = last;
+  lastTag_ = before;
+  *lastTagOut = last;
+  return g;
+}
+
+void Gpu::residueOn(owl_queue q, Buf b) {
   const JobDesc* j = jobs_.get();
+  int wi = b;
+  kResidue(q, 1, 1, 64, 0, {&j, &wi});
+}
+
+u64 Gpu::residueTaken() {
+  int w[64];
+  owlCheck(owl_hip_read(queue_, w, job_.small, sizeof(w), 1), "read residue");
+  return residueFromEnds(w, N, E);
+}
+
+void Gpu::copyOn(owl_queue q, Buf dst, Buf src) {
+  const JobDesc* j = jobs_.get();
+  int d = dst, s = src;
+  kCopy(q, 256, 1, 256, 0, {&j, &d, &s});
+}
+
+u64 Gpu::deviceBytes() const {
+  u64 s = sizeof(JobDesc);
+  for (auto& b : owned_) { s += b->size(); }
+  return s;
+}
+
+u64 Gpu::footprintBound(u32 N, u32 W, u32 H) {
+  const u64 ND = N / 2;
+  // 3 spectra (16 B) + 4 word buffers (8 B) per complex point; carries and stairs
+  // (3 x 8 B per column and block, at most ND); weights; round-off, flags, residue
+  return ND * (3 * 16 + 4 * 8) + 3 * 8 * ND + u64(H) * sizeof(LineW) + u64(W) * sizeof(ColW) +
+         4 * OWL_ROE_SIZE + 4096;
+}
+
+// ---------------------------------------------------------------- launches
+std::string Gpu::shapeKey() const {
+  return rowKernel + "/" + colKernel + "/" + std::to_string(W) + "x" + std::to_string(H) + "/R" + std::to_string(R) +
+         (useLongCarry ? "/long" : "");
+}
+
+void Gpu::rowFused(bool ll, u32 roeSlot, Launch L) {
+  int mode = ROW_FUSED, fi = 0, wi = 0, opts = ll ? 1 : 0;
+  u32 tag = nextStairTag();
+  kRow(L.q, nb + 1, L.n, rowT, rowLds, {&L.jobs, &mode, &fi, &wi, &opts, &roeSlot, &tag});
+}
+
+void Gpu::rowLeadInTo(Buf in, int freqIdx, Launch L) {
   int mode = ROW_LEADIN, fi = freqIdx, wi = in, opts = 0;
   u32 slot = OWL_ROE_SIZE, tag = 0;
-  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &slot, &tag});
+  kRow(L.q, nb, L.n, rowT, rowLds, {&L.jobs, &mode, &fi, &wi, &opts, &slot, &tag});
 }
 
-void Gpu::rowLeadIn(Buf in) { rowLeadInTo(in, 0); }
-
-void Gpu::rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot) {
-  const JobDesc* j = jobs_.get();
+void Gpu::rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot, Launch L) {
   int mode = ROW_CARRYOUT, fi = 0, wi = out, opts = (ll ? 1 : 0) | (mul3 ? 2 : 0);
   u32 tag = 0;
-  kRow(queue_, nb, 1, rowT, R * W * 16, {&j, &mode, &fi, &wi, &opts, &roeSlot, &tag});
+  kRow(L.q, nb, L.n, rowT, rowLds, {&L.jobs, &mode, &fi, &wi, &opts, &roeSlot, &tag});
 }
 
-void Gpu::carryAll(Buf b) {
-  const JobDesc* j = jobs_.get();
+void Gpu::carryAll(Buf b, Launch L) {
   int wi = b, ci = 0, co = 1;
-  kCarryB(queue_, (nb * W + 255) / 256, 1, 256, 0, {&j, &wi, &ci, &co});
-  kCarryFix(queue_, 1, 1, 64, 0, {&j, &wi, &co});
+  kCarryB(L.q, (nb * W + 255) / 256, L.n, 256, 0, {&L.jobs, &wi, &ci, &co});
+  kCarryFix(L.q, 1, L.n, 64, 0, {&L.jobs, &wi, &co});
 }
 
-void Gpu::colSquare() {
-  const JobDesc* j = jobs_.get();
+void Gpu::colSquare(Launch L) {
   int mode = COL_SQUARE, fi = 0, oi = 0;
-  kCol(queue_, W / 2, 1, colT, 2 * H * 16, {&j, &mode, &fi, &oi});
+  kCol(L.q, W / 2, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
 }
 
-void Gpu::colMul() {
-  const JobDesc* j = jobs_.get();
+void Gpu::colMul(Launch L) {
   int mode = COL_MUL, fi = 0, oi = 2;
-  kCol(queue_, W / 2, 1, colT, 2 * H * 16, {&j, &mode, &fi, &oi});
+  kCol(L.q, W / 2, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
 }
 
-void Gpu::colFwdSpec() {
-  const JobDesc* j = jobs_.get();
+void Gpu::colFwdSpec(Launch L) {
   int mode = COL_FWD, fi = 2, oi = 2;
-  kCol(queue_, W / 2, 1, colT, 2 * H * 16, {&j, &mode, &fi, &oi});
+  kCol(L.q, W / 2, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
 }
 
 // ---------------------------------------------------------------- squaring
 void Gpu::square(Buf out, Buf in, bool leadIn, bool leadOut, bool doMul3, bool doLL) {
-  if (leadIn) { rowLeadIn(in); }
-  colSquare();
+  square(out, in, leadIn, leadOut, doMul3, doLL, self());
+}
+
+void Gpu::square(Buf out, Buf in, bool leadIn, bool leadOut, bool doMul3, bool doLL, Launch L) {
+  if (leadIn) { rowLeadInTo(in, 0, L); }
+  colSquare(L);
   if (leadOut) {
-    rowCarryOut(out, doLL, doMul3, nextRoeSlot());
-    carryAll(out);
+    rowCarryOut(out, doLL, doMul3, nextRoeSlot(), L);
+    carryAll(out, L);
   } else {
-    rowFused(doLL, nextRoeSlot());
+    rowFused(doLL, nextRoeSlot(), L);
   }
 }
 
 u32 Gpu::squareLoop(Buf out, Buf in, u32 from, u32 to, bool doTailMul3, bool doLL) {
+  return squareLoop(out, in, from, to, doTailMul3, doLL, self());
+}
+
+u32 Gpu::squareLoop(Buf out, Buf in, u32 from, u32 to, bool doTailMul3, bool doLL, Launch L) {
   bool leadIn = true;
   for (u32 k = from; k < to; ++k) {
     bool leadOut = useLongCarry || k == to - 1;
-    square(out, k == from ? in : out, leadIn, leadOut, doTailMul3 && k == to - 1, doLL);
+    square(out, k == from ? in : out, leadIn, leadOut, doTailMul3 && k == to - 1, doLL, L);
     leadIn = leadOut;
   }
   return to;
 }
 
-void Gpu::modMul(Buf ioA, Buf inB, bool mul3) {
-  rowLeadInTo(inB, 2);
-  colFwdSpec();
-  rowLeadIn(ioA);
-  colMul();
-  rowCarryOut(ioA, false, mul3, nextRoeSlot());
-  carryAll(ioA);
+void Gpu::modMul(Buf ioA, Buf inB, bool mul3) { modMul(ioA, inB, mul3, self()); }
+
+void Gpu::modMul(Buf ioA, Buf inB, bool mul3, Launch L) {
+  rowLeadInTo(inB, 2, L);
+  colFwdSpec(L);
+  rowLeadInTo(ioA, 0, L);
+  colMul(L);
+  rowCarryOut(ioA, false, mul3, nextRoeSlot(), L);
+  carryAll(ioA, L);
+}
+
+void Gpu::injectFault() {
+  faultAt_ = 0;
+  Words w = readData(DATA);
+  w[w.size() / 2] ^= 1u << 7;
+  writeIn(DATA, w);
 }
 
 bool Gpu::doCheck(u32 blockSize) {
@@ -350,8 +447,8 @@ u64 Gpu::dataResidue(Buf b) {
   return residueFromEnds(w, N, E);
 }
 
-RoeInfo Gpu::readROE() {
-  u32 n = std::min<u32>(roePos, OWL_ROE_SIZE);
+RoeInfo Gpu::readROE(u32 nSlots) {
+  u32 n = std::min<u32>(nSlots, OWL_ROE_SIZE);
   RoeInfo r;
   if (!n) { return r; }
   std::vector<uint32_t> bits(n);
@@ -400,7 +497,8 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
     writeIn(CHECK, vCheck);
     u32 k = vK;
     if (resAt && resStep) { resAt->resize(vK / resStep); }
-    bool skip = false, leadIn = true, failed = false;
+    // a verified state already holds x_vK in the check (src/Gpu.cpp:1466-1471, 1545)
+    bool skip = vK > 0, leadIn = true, failed = false;
     while (k < iters) {
       if (skip) {
         skip = false;
@@ -408,9 +506,11 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
         modMul(CHECK, DATA);
       }
       ++k;
-      bool leadOut = (k % B == 0) || k == iters || (resStep && k % resStep == 0) || useLongCarry;
+      bool fault = (k == faultAt_);
+      bool leadOut = (k % B == 0) || k == iters || (resStep && k % resStep == 0) || useLongCarry || fault;
       square(DATA, DATA, leadIn, leadOut);
       leadIn = leadOut;
+      if (fault) { injectFault(); }
       if (resAt && resStep && k % resStep == 0) { resAt->push_back(dataResidue()); }
       bool check = (k % checkStep == 0) || (k == iters && k % B == 0);
       if (check) {

@@ -10,6 +10,7 @@
 #include <hip/hip_vector_types.h>
 
 #include "job.h"
+#include "TrigBufCache.h"
 
 #include <initializer_list>
 #include <memory>
@@ -36,6 +37,7 @@ struct RoeInfo {
 
 struct GpuOptions {
   bool longCarry = false;  // lead-out + lead-in around every iteration (the reference's -carry long)
+  bool generic = false;    // runtime-plan kernels even where a shape-specialized pass exists
   u32 lineBlock = 0;       // lines per row-pass block; 0 = automatic
   int device = 0;
 };
@@ -69,7 +71,8 @@ public:
   // geometry
   const u32 E;
   const FFTConfig fft;
-  u32 N, ND, W, H, R, nb, rowT, colT;
+  u32 N, ND, W, H, R, nb, rowT, colT, rowLds, colLds;
+  std::string rowKernel, colKernel;
   bool useLongCarry;
   std::vector<u32> rowPlan, colPlan;
 
@@ -91,7 +94,9 @@ public:
   bool isEqual(Buf a, Buf b);
   void copy(Buf dst, Buf src);
 
-  RoeInfo readROE();
+  RoeInfo readROE() { return readROE(roePos); }
+  RoeInfo readROE(u32 nSlots);   // the first nSlots round-off slots (then cleared)
+  u32 roeCount() const { return roePos; }
   void finish();
 
   // workflows (src/Gpu.cpp:1337-1663), without savefiles or proofs
@@ -107,28 +112,81 @@ public:
   const JobDesc* jobDev() const { return jobs_.get(); }
   u32 flagsRead();  // bit0 carry leftover, bit1 mismatch, bit2 hand-off timeout
 
-  // kernel-level launches (used by square/modMul; exposed for tests and timing)
-  void rowFused(bool ll, u32 roeSlot);
-  void rowLeadIn(Buf in);
-  void rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot);
-  void colSquare();
-  void colMul();        // state *= spectrum in freq[2]
-  void colFwdSpec();    // freq[2] <- spectrum of freq[2]
-  void rowLeadInTo(Buf in, int freqIdx);
-  void carryAll(Buf b);
+  // Where a pass runs: n jobs of this shape (descriptors in device memory, this Gpu's
+  // geometry and kernels) on queue q. Gpu's own calls use self().
+  struct Launch {
+    const JobDesc* jobs;
+    u32 n;
+    owl_queue q;
+  };
+  Launch self() const { return {jobs_.get(), 1, queue_}; }
+
+  // kernel-level launches (used by square/modMul; exposed for tests, timing and batches)
+  void rowFused(bool ll, u32 roeSlot) { rowFused(ll, roeSlot, self()); }
+  void rowLeadIn(Buf in) { rowLeadInTo(in, 0, self()); }
+  void rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot) { rowCarryOut(out, ll, mul3, roeSlot, self()); }
+  void colSquare() { colSquare(self()); }
+  void colMul() { colMul(self()); }                // state *= spectrum in freq[2]
+  void colFwdSpec() { colFwdSpec(self()); }        // freq[2] <- spectrum of freq[2]
+  void rowLeadInTo(Buf in, int freqIdx) { rowLeadInTo(in, freqIdx, self()); }
+  void carryAll(Buf b) { carryAll(b, self()); }
+
+  void rowFused(bool ll, u32 roeSlot, Launch L);
+  void rowLeadInTo(Buf in, int freqIdx, Launch L);
+  void rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot, Launch L);
+  void colSquare(Launch L);
+  void colMul(Launch L);
+  void colFwdSpec(Launch L);
+  void carryAll(Buf b, Launch L);
+  void square(Buf out, Buf in, bool leadIn, bool leadOut, bool doMul3, bool doLL, Launch L);
+  u32 squareLoop(Buf out, Buf in, u32 from, u32 to, bool doTailMul3, bool doLL, Launch L);
+  void modMul(Buf ioA, Buf inB, bool mul3, Launch L);
+  u32 nextRoeSlot();
+  void resetRoeCount() { roePos = 0; }
+  // A tag for the next fused launch: never the tag of the fused launch before it (noteTag)
+  // nor one of a captured graph's tags (reserveTags), so that consecutive launches on a
+  // job's carry stairs always differ, also around graph replays and re-captures.
+  u32 nextStairTag();
+  void noteTag(u32 t) { lastTag_ = t; }
+  u32 lastTag() const { return lastTag_; }
+  void reserveTags(u32 lo, u32 hi) { resLo_ = lo, resHi_ = hi; }
+  // take over another engine's tag walk (a group's launches moving to a new lead)
+  void inheritTags(const Gpu& o) {
+    stairTag_ = o.stairTag_, lastTag_ = o.lastTag_, resLo_ = o.resLo_, resHi_ = o.resHi_;
+  }
+  // G fused squarings (column pass + fused row pass) captured as one graph on L.q;
+  // its tags are reserved until the next capture. Returns the graph and its last tag.
+  owl_graph captureFused(u32 G, bool ll, u32 roeSlot, Launch L, u32* lastTagOut);
+  // res64 of b read on queue q into this job's small buffer; residueTaken() converts it
+  // after q has finished (the two halves of dataResidue, for launches shared by a group)
+  void residueOn(owl_queue q, Buf b);
+  u64 residueTaken();
+  void copyOn(owl_queue q, Buf dst, Buf src);
+  // device bytes this engine holds (state, Gerbicz buffers, carries, tables of its own)
+  u64 deviceBytes() const;
+  // an upper bound of deviceBytes() for an exponent at FFT size N, before allocating
+  static u64 footprintBound(u32 N, u32 W, u32 H);
+  // the shape key of a batch group: jobs with equal keys can share launches
+  std::string shapeKey() const;
+  // Failure-handling tests: the PRP loops flip one bit of the data right after iteration k
+  // (once), as a hardware error would; the next Gerbicz check must catch it.
+  void armFault(u32 k) { faultAt_ = k; }
+  u32 faultAt() const { return faultAt_; }
+  void injectFault();
 
 private:
   owl_queue queue_ = nullptr;
   JobDesc job_{};
   Buffer<JobDesc> jobs_;
   std::vector<std::unique_ptr<Buffer<char>>> owned_;
+  std::shared_ptr<const TrigTables> trig_;
   u32 roePos = 0;
-  u32 stairTag_ = 0;
+  u32 stairTag_ = 0, lastTag_ = 0, resLo_ = 1, resHi_ = 0;
+  u32 faultAt_ = 0;
   Kernel kRow, kCol, kCarryB, kCarryFix, kWordsIn, kWordsOut, kResidue, kIsEqual, kSum64, kCopy;
 
   template<typename T> T* alloc(size_t n);
   template<typename T> T* upload(const std::vector<T>& v);
-  u32 nextRoeSlot();
 };
 
 // Mersenne-prime type-1 PRP residue is 9 (src/Gpu.cpp:1036-1040).

@@ -1,0 +1,349 @@
+// Request loop with continuous batching; see Server.h.
+#include "Server.h"
+
+#include <algorithm>
+#include <chrono>
+#include <stdexcept>
+
+namespace {
+double steadySecs() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+Words small(u32 E, u32 v) {
+  Words w(nWords(E));
+  w[0] = v;
+  return w;
+}
+}  // namespace
+
+Server::Server(const Options& opt) : opt_{opt}, t0_{steadySecs()} {
+  if (opt_.blockSize < 2) { throw std::invalid_argument("block size must be at least 2"); }
+  if (!opt_.checkStep) { opt_.checkStep = 50 * opt_.blockSize; }
+  if (opt_.checkStep % opt_.blockSize) { throw std::invalid_argument("check step must be a multiple of the block size"); }
+  if (!opt_.maxJobs) { throw std::invalid_argument("maxJobs"); }
+  if (!opt_.memBudget) {
+    size_t total = 0, freeMem = 0;
+    int cu = 0, lds = 0;
+    owlCheck(owl_hip_device_info(opt_.gpu.device, nullptr, 0, &total, &freeMem, &cu, &lds), "device info");
+    opt_.memBudget = freeMem / 4 * 3;
+  }
+}
+
+Server::~Server() {
+  for (auto& [key, g] : groups_) {
+    if (g->lead) { owl_hip_finish(g->lead->queue()); }
+    if (g->graph) { owl_hip_graph_destroy(g->graph); }
+  }
+  groups_.clear();
+  active_.clear();
+}
+
+double Server::now() const { return steadySecs() - t0_; }
+
+u32 Server::submit(u32 E, bool ll, u32 iters) {
+  if (!iters) { throw std::invalid_argument("a request needs at least one iteration"); }
+  if (ll && iters > E - 2) { throw std::invalid_argument("LL runs at most E - 2 iterations"); }
+  if (!ll && iters > E) { throw std::invalid_argument("PRP runs at most E iterations"); }
+  FFTConfig fft = FFTConfig::bestFit(E, opt_.spec);  // throws for an exponent no FFT covers
+  if (double(E) / fft.size() < FFTShape::MIN_BPW || E % 32 == 0 || E < 64) {
+    throw std::invalid_argument("exponent " + std::to_string(E) + " is too small for FFT " + fft.spec());
+  }
+  u32 id = nextId_++;
+  waiting_.push_back({id, E, iters, ll, now()});
+  return id;
+}
+
+Server::Group& Server::groupOf(Job& j) {
+  std::string key = j.gpu->shapeKey() + (j.req.ll ? "/ll" : "/prp");
+  auto& slot = groups_[key];
+  if (!slot) {
+    slot = std::make_unique<Group>();
+    slot->key = key;
+    slot->ll = j.req.ll;
+  }
+  return *slot;
+}
+
+void Server::admit() {
+  for (auto it = waiting_.begin(); it != waiting_.end() && active_.size() < opt_.maxJobs;) {
+    const Request& r = *it;
+    FFTConfig fft = FFTConfig::bestFit(r.E, opt_.spec);
+    u32 N = fft.size(), W = fft.shape.width, H = fft.shape.middle * fft.shape.height;
+    if (bytesInUse_ + Gpu::footprintBound(N, W, H) > opt_.memBudget) {
+      ++it;  // first fit: a smaller request behind may still fit
+      continue;
+    }
+    auto job = std::make_unique<Job>();
+    job->req = r;
+    job->res.id = r.id;
+    job->res.E = r.E;
+    job->res.iters = r.iters;
+    job->res.ll = r.ll;
+    job->res.tSubmit = r.tSubmit;
+    job->res.tAdmit = now();
+    try {
+      job->gpu = std::make_unique<Gpu>(r.E, fft, opt_.gpu);
+    } catch (const std::invalid_argument&) {
+      job->res.status = 2;  // rejected: no engine for this exponent
+      job->res.tDone = now();
+      done_.push_back(job->res);
+      ++nDone_;
+      it = waiting_.erase(it);
+      continue;
+    }
+    Gpu& g = *job->gpu;
+    job->bytes = g.deviceBytes();
+    job->kEndEnd = (r.iters + opt_.blockSize - 1) / opt_.blockSize * opt_.blockSize;
+    if (r.ll) {
+      g.writeIn(Gpu::DATA, small(r.E, 4));
+    } else {
+      job->vData = small(r.E, 3);
+      job->vCheck = small(r.E, 1);
+      g.writeIn(Gpu::DATA, job->vData);
+      g.writeIn(Gpu::CHECK, job->vCheck);
+    }
+    g.readROE(OWL_ROE_SIZE);
+    bytesInUse_ += job->bytes;
+    Group& grp = groupOf(*job);
+    if (!grp.lead) { grp.lead = &g; }
+    grp.members.push_back(job.get());
+    grp.dirty = true;
+    active_.push_back(std::move(job));
+    it = waiting_.erase(it);
+  }
+}
+
+void Server::prepare(Group& g) {
+  const u32 n = u32(g.members.size());
+  if (n > g.cap) {
+    g.cap = std::max<u32>({n, 2 * g.cap, 8});
+    g.all = Buffer<JobDesc>(g.cap);
+    g.sub = Buffer<JobDesc>(g.cap);
+  }
+  std::vector<JobDesc> d;
+  for (Job* j : g.members) { d.push_back(j->gpu->job()); }
+  g.all.write(g.lead->queue(), d);
+  if (g.graph) {
+    owlCheck(owl_hip_graph_destroy(g.graph), "graph destroy");
+    g.graph = nullptr;
+  }
+  if (!g.lead->useLongCarry) { g.graph = g.lead->captureFused(GRAPH_LEN, g.ll, 0, g.launch(), &g.graphLast); }
+  g.dirty = false;
+}
+
+Gpu::Launch Server::subset(Group& g, const std::vector<Job*>& js) {
+  if (js.size() == g.members.size()) { return g.launch(); }
+  std::vector<JobDesc> d;
+  for (Job* j : js) { d.push_back(j->gpu->job()); }
+  g.sub.write(g.lead->queue(), d);  // blocking: the queue is idle between rounds
+  return {g.sub.get(), u32(js.size()), g.lead->queue()};
+}
+
+// count fused squarings (column pass + fused row pass), from captured graphs where possible
+void Server::fused(Group& g, u32 count) {
+  Gpu& L = *g.lead;
+  Gpu::Launch ln = g.launch();
+  u32 i = 0;
+  for (; g.graph && i + GRAPH_LEN <= count; i += GRAPH_LEN) {
+    owlCheck(owl_hip_graph_launch(g.graph, ln.q), "graph launch");
+    L.noteTag(g.graphLast);
+  }
+  for (; i < count; ++i) {
+    L.colSquare(ln);
+    L.rowFused(g.ll, 0, ln);
+  }
+}
+
+void Server::runRound(Group& g) {
+  Gpu& L = *g.lead;
+  const u32 B = opt_.blockSize;
+  Gpu::Launch ln = g.launch();
+
+  // Gerbicz block update: check *= data (src/Gpu.cpp:1466-1471)
+  if (!g.ll) {
+    std::vector<Job*> mul;
+    for (Job* j : g.members) {
+      if (j->skip) {
+        j->skip = false;
+      } else {
+        mul.push_back(j);
+      }
+    }
+    if (!mul.empty()) {
+      Gpu::Launch ls = subset(g, mul);
+      L.modMul(Gpu::CHECK, Gpu::DATA, false, ls);
+    }
+  }
+
+  // segment ends: the requested iteration of every job that reaches it in this round
+  std::vector<u32> cuts;
+  for (Job* j : g.members) {
+    if (!j->captured && j->req.iters > j->k && j->req.iters <= j->k + B) { cuts.push_back(j->req.iters - j->k); }
+  }
+  cuts.push_back(B);
+  std::sort(cuts.begin(), cuts.end());
+  cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+
+  u32 pos = 0;
+  for (u32 cut : cuts) {
+    const u32 n = cut - pos;
+    if (L.useLongCarry) {
+      for (u32 i = 0; i < n; ++i) { L.square(Gpu::DATA, Gpu::DATA, true, true, false, g.ll, ln); }
+    } else {
+      L.rowLeadInTo(Gpu::DATA, 0, ln);
+      fused(g, n - 1);
+      L.colSquare(ln);
+      L.rowCarryOut(Gpu::DATA, g.ll, false, 0, ln);
+      L.carryAll(Gpu::DATA, ln);
+    }
+    for (Job* j : g.members) {
+      if (!j->captured && j->req.iters == j->k + cut) {
+        j->gpu->residueOn(ln.q, Gpu::DATA);
+        bool final = j->req.iters == (g.ll ? j->req.E - 2 : j->req.E);
+        if (final) { j->gpu->copyOn(ln.q, Gpu::BASE, Gpu::DATA); }
+        j->captured = true;
+      }
+    }
+    pos = cut;
+  }
+  for (Job* j : g.members) {
+    u32 useful = j->k < j->req.iters ? std::min(B, j->req.iters - j->k) : 0;
+    iterations_ += useful;
+    wordIterations_ += u64(useful) * j->gpu->N;
+    j->k += B;
+    if (!g.ll) { j->wantCheck = (j->k % opt_.checkStep == 0) || j->k >= j->kEndEnd; }
+  }
+}
+
+// Gerbicz verification of the jobs due for it (src/Gpu.cpp:767-771)
+void Server::verify(Group& g) {
+  std::vector<Job*> due;
+  for (Job* j : g.members) {
+    if (j->wantCheck) { due.push_back(j); }
+  }
+  if (due.empty()) { return; }
+  Gpu::Launch ls = subset(g, due);
+  g.lead->squareLoop(Gpu::AUX, Gpu::CHECK, 0, opt_.blockSize, true, false, ls);
+  g.lead->modMul(Gpu::CHECK, Gpu::DATA, false, ls);
+}
+
+void Server::retire() {
+  std::vector<std::unique_ptr<Job>> keep;
+  for (auto& jp : active_) {
+    Job& j = *jp;
+    bool finished = j.k >= j.kEndEnd && j.captured && j.res.status == 0;
+    if (!finished && j.res.status == 0) {
+      keep.push_back(std::move(jp));
+      continue;
+    }
+    j.res.tDone = now();
+    done_.push_back(j.res);
+    ++nDone_;
+    Group& g = groupOf(j);
+    g.members.erase(std::find(g.members.begin(), g.members.end(), &j));
+    g.dirty = true;
+    if (g.lead == j.gpu.get()) {
+      Gpu* next = g.members.empty() ? nullptr : g.members.front()->gpu.get();
+      if (next) { next->inheritTags(*g.lead); }
+      if (g.graph) {
+        owlCheck(owl_hip_graph_destroy(g.graph), "graph destroy");
+        g.graph = nullptr;
+      }
+      g.lead = next;
+    }
+    bytesInUse_ -= j.bytes;
+    if (g.members.empty()) {
+      std::string key = g.key;
+      groups_.erase(key);
+    }
+  }
+  active_ = std::move(keep);
+}
+
+u32 Server::step() {
+  admit();
+  std::vector<Group*> gs;
+  for (auto& [key, g] : groups_) {
+    if (g->dirty) { prepare(*g); }
+    gs.push_back(g.get());
+  }
+  for (Group* g : gs) { runRound(*g); }
+  for (Group* g : gs) { g->lead->finish(); }
+
+  // residues taken in this round
+  for (Group* g : gs) {
+    for (Job* j : g->members) {
+      if (j->captured && !j->resRead) {
+        j->resRead = true;
+        j->res.res64 = j->gpu->residueTaken();
+        if (j->req.iters == (g->ll ? j->req.E - 2 : j->req.E)) {
+          Words w = j->gpu->readData(Gpu::BASE);
+          j->res.isPrime = g->ll ? w.empty() : equals9(w);
+        }
+      }
+    }
+  }
+
+  for (Group* g : gs) {
+    if (!g->ll) { verify(*g); }
+  }
+  for (Group* g : gs) { g->lead->finish(); }
+  for (Group* g : gs) {
+    const u32 nSlots = std::max<u32>(1, std::min<u32>(g->lead->roeCount(), OWL_ROE_SIZE));
+    for (Job* j : g->members) { j->res.maxRoe = std::max(j->res.maxRoe, j->gpu->readROE(nSlots).max); }
+    g->lead->resetRoeCount();
+    for (Job* j : g->members) {
+      if (!j->wantCheck) { continue; }
+      j->wantCheck = false;
+      if (j->gpu->isEqual(Gpu::CHECK, Gpu::AUX)) {
+        ++j->res.nChecks;
+        j->skip = true;
+        j->vK = j->k;
+        if (j->k < j->kEndEnd) {
+          j->vData = j->gpu->readData(Gpu::DATA);
+          j->vCheck = j->gpu->readData(Gpu::CHECK);
+        }
+      } else {
+        // reload the last verified state (src/Gpu.cpp:1561-1575)
+        if (++j->res.nErrors > 3) {
+          j->res.status = 1;
+          continue;
+        }
+        j->gpu->writeIn(Gpu::DATA, j->vData);
+        j->gpu->writeIn(Gpu::CHECK, j->vCheck);
+        if (j->vK < j->req.iters) {
+          j->captured = j->resRead = false;
+          j->res.res64 = 0;
+          j->res.isPrime = false;
+        }
+        j->k = j->vK;
+        j->skip = j->vK > 0;
+      }
+    }
+  }
+  u32 before = nDone_;
+  retire();
+  ++rounds_;
+  return nDone_ - before;
+}
+
+std::vector<ServeResult> Server::take() {
+  std::vector<ServeResult> out;
+  out.swap(done_);
+  return out;
+}
+
+ServeInfo Server::info() const {
+  ServeInfo s;
+  s.waiting = u32(waiting_.size());
+  s.active = u32(active_.size());
+  s.groups = u32(groups_.size());
+  s.done = nDone_;
+  s.bytesInUse = bytesInUse_;
+  s.budget = opt_.memBudget;
+  s.iterations = iterations_;
+  s.wordIterations = wordIterations_;
+  s.rounds = rounds_;
+  return s;
+}

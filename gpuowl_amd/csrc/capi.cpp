@@ -2,6 +2,10 @@
 #include "owl.h"
 
 #include "Gpu.h"
+#include "GpuBatch.h"
+#include "Server.h"
+
+#include <deque>
 
 #include <cstring>
 #include <exception>
@@ -9,6 +13,13 @@
 
 struct owl_gpu_s {
   Gpu* gpu;
+};
+struct owl_batch_s {
+  GpuBatch* batch;
+};
+struct owl_server_s {
+  Server* server;
+  std::deque<ServeResult> ready;
 };
 
 // Shares the thread-local error slot of the device layer.
@@ -49,12 +60,13 @@ int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen,
   });
 }
 
-int owl_gpu_create(uint32_t E, const char* spec, int longCarry, uint32_t lineBlock, owl_gpu* out) {
+int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, owl_gpu* out) {
   *out = nullptr;
   return guarded([&] {
     FFTConfig fft = FFTConfig::bestFit(E, spec ? spec : "");
     GpuOptions opt;
-    opt.longCarry = longCarry != 0;
+    opt.longCarry = flags & OWL_LONG_CARRY;
+    opt.generic = flags & OWL_GENERIC;
     opt.lineBlock = lineBlock;
     *out = new owl_gpu_s{new Gpu(E, fft, opt)};
   });
@@ -69,9 +81,14 @@ int owl_gpu_destroy(owl_gpu g) {
   });
 }
 
+void fillInfo(const Gpu& G, owl_gpu_info_t* info);
+
 int owl_gpu_info(owl_gpu g, owl_gpu_info_t* info) {
-  return guarded([&] {
-    Gpu& G = *g->gpu;
+  return guarded([&] { fillInfo(*g->gpu, info); });
+}
+
+void fillInfo(const Gpu& G, owl_gpu_info_t* info) {
+  {
     memset(info, 0, sizeof(*info));
     info->E = G.E;
     info->N = G.N;
@@ -84,7 +101,9 @@ int owl_gpu_info(owl_gpu g, owl_gpu_info_t* info) {
     info->maxExp = G.fft.maxExp();
     info->longCarry = G.useLongCarry;
     snprintf(info->spec, sizeof(info->spec), "%s", G.fft.spec().c_str());
-  });
+    snprintf(info->rowKernel, sizeof(info->rowKernel), "%s", G.rowKernel.c_str());
+    snprintf(info->colKernel, sizeof(info->colKernel), "%s", G.colKernel.c_str());
+  }
 }
 
 int owl_gpu_write_words(owl_gpu g, int b, const uint32_t* words, uint32_t n) {
@@ -224,6 +243,194 @@ int owl_gpu_time_kernels(owl_gpu g, uint32_t iters, double* usCol, double* usRow
     G.carryAll(Gpu::DATA);
     G.finish();
   });
+}
+
+int owl_gpu_time_copy(owl_gpu g, uint32_t iters, double* us) {
+  return guarded([&] {
+    Gpu& G = *g->gpu;
+    owl_queue q = G.queue();
+    size_t bytes = size_t(G.ND) * 16;
+    owlCheck(owl_hip_copy(q, G.job().freq[1], G.job().freq[0], bytes), "copy");
+    owl_event a, b;
+    owlCheck(owl_hip_event_create(&a), "event");
+    owlCheck(owl_hip_event_create(&b), "event");
+    owlCheck(owl_hip_event_record(a, q), "event");
+    for (uint32_t i = 0; i < iters; ++i) {
+      owlCheck(owl_hip_copy(q, G.job().freq[1 + (i & 1)], G.job().freq[0], bytes), "copy");
+    }
+    owlCheck(owl_hip_event_record(b, q), "event");
+    float ms = 0;
+    owlCheck(owl_hip_event_elapsed_ms(a, b, &ms), "elapsed");
+    owl_hip_event_destroy(a);
+    owl_hip_event_destroy(b);
+    *us = ms * 1e3 / iters;
+  });
+}
+
+int owl_server_create(uint64_t memBudget, uint32_t maxJobs, uint32_t blockSize, uint32_t checkStep, const char* spec,
+                      int flags, owl_server* out) {
+  *out = nullptr;
+  return guarded([&] {
+    Server::Options o;
+    o.memBudget = memBudget;
+    o.maxJobs = maxJobs;
+    o.blockSize = blockSize;
+    o.checkStep = checkStep;
+    o.spec = spec ? spec : "";
+    o.gpu.longCarry = flags & OWL_LONG_CARRY;
+    o.gpu.generic = flags & OWL_GENERIC;
+    *out = new owl_server_s{new Server(o), {}};
+  });
+}
+
+int owl_server_destroy(owl_server s) {
+  return guarded([&] {
+    if (s) {
+      delete s->server;
+      delete s;
+    }
+  });
+}
+
+int owl_server_submit(owl_server s, uint32_t E, int ll, uint32_t iters, uint32_t* id) {
+  return guarded([&] { *id = s->server->submit(E, ll != 0, iters); });
+}
+
+int owl_server_step(owl_server s, uint32_t* nDone) {
+  return guarded([&] {
+    u32 n = s->server->step();
+    for (ServeResult& r : s->server->take()) { s->ready.push_back(r); }
+    if (nDone) { *nDone = n; }
+  });
+}
+
+int owl_server_take(owl_server s, owl_serve_result_t* out, uint32_t cap, uint32_t* n) {
+  return guarded([&] {
+    u32 i = 0;
+    for (; i < cap && !s->ready.empty(); ++i) {
+      const ServeResult& r = s->ready.front();
+      owl_serve_result_t& o = out[i];
+      o.id = r.id;
+      o.E = r.E;
+      o.iters = r.iters;
+      o.ll = r.ll;
+      o.res64 = r.res64;
+      o.isPrime = r.isPrime;
+      o.nErrors = r.nErrors;
+      o.nChecks = r.nChecks;
+      o.maxRoe = r.maxRoe;
+      o.tSubmit = r.tSubmit;
+      o.tAdmit = r.tAdmit;
+      o.tDone = r.tDone;
+      o.status = r.status;
+      s->ready.pop_front();
+    }
+    *n = i;
+  });
+}
+
+int owl_server_info(owl_server s, owl_serve_info_t* info) {
+  return guarded([&] {
+    ServeInfo x = s->server->info();
+    info->waiting = x.waiting;
+    info->active = x.active;
+    info->groups = x.groups;
+    info->done = x.done;
+    info->rounds = x.rounds;
+    info->bytesInUse = x.bytesInUse;
+    info->budget = x.budget;
+    info->iterations = x.iterations;
+    info->wordIterations = x.wordIterations;
+  });
+}
+
+int owl_server_now(owl_server s, double* secs) {
+  return guarded([&] { *secs = s->server->now(); });
+}
+
+int owl_batch_create(const uint32_t* exponents, uint32_t n, const char* spec, int flags, owl_batch* out) {
+  *out = nullptr;
+  return guarded([&] {
+    GpuOptions opt;
+    opt.longCarry = flags & OWL_LONG_CARRY;
+    opt.generic = flags & OWL_GENERIC;
+    *out = new owl_batch_s{new GpuBatch(std::vector<u32>(exponents, exponents + n), spec ? spec : "", opt)};
+  });
+}
+
+int owl_batch_destroy(owl_batch b) {
+  return guarded([&] {
+    if (b) {
+      delete b->batch;
+      delete b;
+    }
+  });
+}
+
+int owl_batch_info(owl_batch b, uint32_t* nJobs, uint32_t* nGroups, uint64_t* totalWords) {
+  return guarded([&] {
+    *nJobs = u32(b->batch->size());
+    *nGroups = u32(b->batch->groups.size());
+    *totalWords = b->batch->totalWords();
+  });
+}
+
+int owl_gpu_arm_fault(owl_gpu g, uint32_t k) {
+  return guarded([&] { g->gpu->armFault(k); });
+}
+
+int owl_batch_arm_fault(owl_batch b, uint32_t job, uint32_t k) {
+  return guarded([&] {
+    if (job >= b->batch->size()) { throw std::invalid_argument("job index"); }
+    b->batch->jobs[job]->armFault(k);
+  });
+}
+
+int owl_batch_job_info(owl_batch b, uint32_t job, owl_gpu_info_t* info) {
+  return guarded([&] {
+    if (job >= b->batch->size()) { throw std::invalid_argument("job index"); }
+    fillInfo(*b->batch->jobs[job], info);
+  });
+}
+
+int owl_batch_prp(owl_batch b, uint32_t iters, uint32_t blockSize, uint32_t checkStep, uint64_t* res, uint32_t resStep,
+                  uint32_t resCap, owl_prp_result_t* out) {
+  return guarded([&] {
+    std::vector<std::vector<u64>> at;
+    auto r = b->batch->prp(iters, blockSize, checkStep, res ? &at : nullptr, res ? resStep : 0);
+    for (size_t i = 0; i < r.size(); ++i) {
+      if (res) {
+        for (size_t k = 0; k < at[i].size() && k < resCap; ++k) { res[i * resCap + k] = at[i][k]; }
+      }
+      out[i].res64 = r[i].res64;
+      out[i].k = r[i].k;
+      out[i].nErrors = r[i].nErrors;
+      out[i].nChecks = r[i].nChecks;
+      out[i].isPrime = r[i].isPrime;
+      out[i].maxRoe = r[i].maxRoe;
+      out[i].secsPerIt = r[i].secsPerIt;
+    }
+  });
+}
+
+int owl_batch_ll(owl_batch b, uint32_t iters, uint64_t* res, uint32_t resStep, uint32_t resCap, owl_ll_result_t* out) {
+  return guarded([&] {
+    std::vector<std::vector<u64>> at;
+    auto r = b->batch->ll(iters, res ? &at : nullptr, res ? resStep : 0);
+    for (size_t i = 0; i < r.size(); ++i) {
+      if (res) {
+        for (size_t k = 0; k < at[i].size() && k < resCap; ++k) { res[i * resCap + k] = at[i][k]; }
+      }
+      out[i].res64 = r[i].res64;
+      out[i].k = r[i].k;
+      out[i].isPrime = r[i].isPrime;
+      out[i].secsPerIt = r[i].secsPerIt;
+    }
+  });
+}
+
+int owl_batch_time_squares(owl_batch b, uint32_t iters, uint32_t warmup, int useGraph, double* us) {
+  return guarded([&] { *us = b->batch->timeSquares(iters, warmup, useGraph != 0); });
 }
 
 int owl_gpu_freq_io(owl_gpu g, int idx, double* host, uint32_t n, int write) {

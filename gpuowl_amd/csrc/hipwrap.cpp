@@ -18,6 +18,7 @@ struct OwlKernelEntry {
   unsigned maxThreads;
 };
 extern "C" const OwlKernelEntry owl_kernel_table[];
+extern "C" const OwlKernelEntry owl_kernel_table_fixed[];
 
 struct owl_queue_s {
   hipStream_t stream;
@@ -151,12 +152,14 @@ int owl_hip_kernel_get(const char* name, owl_kernel* kernel) {
     *kernel = it->second;
     return 0;
   }
-  for (const OwlKernelEntry* e = owl_kernel_table; e->name; ++e) {
-    if (!strcmp(e->name, name)) {
-      auto* k = new owl_kernel_s{*e};
-      cache[name] = k;
-      *kernel = k;
-      return 0;
+  for (const OwlKernelEntry* table : {owl_kernel_table, owl_kernel_table_fixed}) {
+    for (const OwlKernelEntry* e = table; e->name; ++e) {
+      if (!strcmp(e->name, name)) {
+        auto* k = new owl_kernel_s{*e};
+        cache[name] = k;
+        *kernel = k;
+        return 0;
+      }
     }
   }
   *kernel = nullptr;

@@ -45,6 +45,7 @@ struct JobDesc {
   uint32_t bitsLo;                  // E / N
   uint32_t rE;                      // E % N: word k is "big" iff extra(k) < rE
   uint32_t nHi;                     // ceil(H / 64): rows of the split mid twiddle table
+  uint32_t lgR;                     // log2(R)
   uint32_t nRow, nCol;              // number of FFT stages
   uint32_t rowPlan[OWL_MAX_STAGES]; // radices of the W-point row FFT
   uint32_t colPlan[OWL_MAX_STAGES]; // radices of the H-point column FFT
@@ -54,6 +55,7 @@ struct JobDesc {
   const double2* twMidA;  // W*64: exp(-2 pi i a*lo / ND), lo < 64
   const double2* twMidB;  // W*nHi: exp(-2 pi i a*64*hi / ND)
   const double2* twPair;  // W entries: exp(-2 pi i a / ND)
+  const double2* twMidS;  // W*16: exp(-2 pi i a*k*(H/16) / ND), k < 16 (shape-specialized column pass)
   const LineW* lineW;     // H entries
   const ColW* colW;       // W entries
 

@@ -2,7 +2,7 @@
 src/Gpu.h:213-300, src/FFTConfig.h:69-87) over the C-ABI in include/owl.h."""
 import ctypes as C
 
-from ._lib import GpuInfo, LLResult, OwlError, PrpResult, check, load
+from ._lib import GpuInfo, LLResult, OwlError, PrpResult, ServeInfo, ServeResult, check, load
 
 DATA, AUX, CHECK, BASE = 0, 1, 2, 3
 
@@ -29,17 +29,19 @@ def int_to_words(x, E):
 
 
 class Gpu:
-    """One exponent on the GPU: Gpu(E, spec="", long_carry=False, line_block=0)."""
+    """One exponent on the GPU: Gpu(E, spec="", long_carry=False, line_block=0, generic=False)."""
 
-    def __init__(self, E, spec="", long_carry=False, line_block=0):
+    def __init__(self, E, spec="", long_carry=False, line_block=0, generic=False):
         self.lib = load()
         self.h = C.c_void_p()
-        check(self.lib.owl_gpu_create(E, (spec or "").encode(), int(long_carry), line_block, C.byref(self.h)))
+        flags = (1 if long_carry else 0) | (2 if generic else 0)
+        check(self.lib.owl_gpu_create(E, (spec or "").encode(), flags, line_block, C.byref(self.h)))
         info = GpuInfo()
         check(self.lib.owl_gpu_info(self.h, C.byref(info)))
         self.E, self.N, self.W, self.H, self.R = info.E, info.N, info.W, info.H, info.R
         self.nb, self.row_threads, self.col_threads = info.nb, info.rowThreads, info.colThreads
         self.spec, self.max_exp, self.long_carry = info.spec.decode(), info.maxExp, bool(info.longCarry)
+        self.row_kernel, self.col_kernel = info.rowKernel.decode(), info.colKernel.decode()
 
     def close(self):
         if self.h:
@@ -116,6 +118,10 @@ class Gpu:
         return {"n": n.value, "max": mx.value, "mean": mean.value, "sd": sd.value}
 
     # ---- workflows
+    def arm_fault(self, k):
+        """The next prp() flips one data bit right after iteration k (failure-handling tests)."""
+        check(self.lib.owl_gpu_arm_fault(self.h, k))
+
     def prp(self, iters, block_size=1000, check_step=0, res_step=0):
         cap = iters // res_step if res_step else 0
         res = (C.c_uint64 * max(cap, 1))()
@@ -159,11 +165,156 @@ class Gpu:
     def run_pass(self, which, arg=0):
         check(self.lib.owl_gpu_pass(self.h, which, arg))
 
+    def time_copy(self, iters):
+        us = C.c_double()
+        check(self.lib.owl_gpu_time_copy(self.h, iters, C.byref(us)))
+        return us.value
+
     def time_kernels(self, iters):
         a, b = C.c_double(), C.c_double()
         check(self.lib.owl_gpu_time_kernels(self.h, iters, C.byref(a), C.byref(b)))
         return a.value, b.value
 
 
-__all__ = ["Gpu", "best_fit", "OwlError", "DATA", "AUX", "CHECK", "BASE", "n_words", "words_to_int",
+class Batch:
+    """Many exponents advanced together: Batch(exponents, spec="", long_carry=False, generic=False).
+    Jobs of equal FFT geometry share every launch."""
+
+    def __init__(self, exponents, spec="", long_carry=False, generic=False):
+        self.lib = load()
+        self.h = C.c_void_p()
+        arr = (C.c_uint32 * len(exponents))(*exponents)
+        flags = (1 if long_carry else 0) | (2 if generic else 0)
+        check(self.lib.owl_batch_create(arr, len(exponents), (spec or "").encode(), flags, C.byref(self.h)))
+        n, ng, tw = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(self.lib.owl_batch_info(self.h, C.byref(n), C.byref(ng), C.byref(tw)))
+        self.n, self.n_groups, self.total_words = n.value, ng.value, tw.value
+        self.exponents = list(exponents)
+
+    def job_info(self, i):
+        info = GpuInfo()
+        check(self.lib.owl_batch_job_info(self.h, i, C.byref(info)))
+        return {"E": info.E, "N": info.N, "spec": info.spec.decode(), "row": info.rowKernel.decode(),
+                "col": info.colKernel.decode(), "R": info.R, "long_carry": bool(info.longCarry)}
+
+    def arm_fault(self, job, k):
+        check(self.lib.owl_batch_arm_fault(self.h, job, k))
+
+    def close(self):
+        if self.h:
+            check(self.lib.owl_batch_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prp(self, iters, block_size=1000, check_step=0, res_step=0):
+        cap = iters // res_step if res_step else 0
+        res = (C.c_uint64 * max(1, self.n * cap))()
+        out = (PrpResult * self.n)()
+        check(self.lib.owl_batch_prp(self.h, iters, block_size, check_step, res if res_step else None, res_step, cap,
+                                     out))
+        rows = [list(res)[i * cap:(i + 1) * cap] for i in range(self.n)]
+        return list(out), rows
+
+    def ll(self, iters, res_step=0):
+        cap = iters // res_step if res_step else 0
+        res = (C.c_uint64 * max(1, self.n * cap))()
+        out = (LLResult * self.n)()
+        check(self.lib.owl_batch_ll(self.h, iters, res if res_step else None, res_step, cap, out))
+        rows = [list(res)[i * cap:(i + 1) * cap] for i in range(self.n)]
+        return list(out), rows
+
+    def time_squares(self, iters, warmup=10, use_graph=False):
+        us = C.c_double()
+        check(self.lib.owl_batch_time_squares(self.h, iters, warmup, int(use_graph), C.byref(us)))
+        return us.value
+
+
+__all__ = ["Gpu", "Batch", "best_fit", "OwlError", "DATA", "AUX", "CHECK", "BASE", "n_words", "words_to_int",
            "int_to_words"]
+
+
+class Server:
+    """Request loop with continuous batching (include/owl.h, owl_server_*): submit PRP/LL
+    requests at any time; each step() admits what fits, runs one round of block_size
+    iterations on every active job and returns the results finished in it."""
+
+    def __init__(self, mem_budget=0, max_jobs=256, block_size=1000, check_step=50_000, spec="", long_carry=False,
+                 generic=False):
+        self.lib = load()
+        self.h = C.c_void_p()
+        flags = (1 if long_carry else 0) | (2 if generic else 0)
+        check(self.lib.owl_server_create(mem_budget, max_jobs, block_size, check_step, (spec or "").encode(), flags,
+                                         C.byref(self.h)))
+        self.block_size = block_size
+
+    def submit(self, E, kind="prp", iters=None):
+        ll = kind == "ll"
+        if iters is None:
+            iters = E - 2 if ll else E
+        rid = C.c_uint32()
+        check(self.lib.owl_server_submit(self.h, E, 1 if ll else 0, iters, C.byref(rid)))
+        return rid.value
+
+    def step(self):
+        n = C.c_uint32()
+        check(self.lib.owl_server_step(self.h, C.byref(n)))
+        return self.take()
+
+    def take(self):
+        out = []
+        buf = (ServeResult * 64)()
+        while True:
+            n = C.c_uint32()
+            check(self.lib.owl_server_take(self.h, buf, 64, C.byref(n)))
+            for r in buf[:n.value]:
+                out.append({f: getattr(r, f) for f, _ in ServeResult._fields_})
+            if n.value < 64:
+                return out
+
+    def info(self):
+        i = ServeInfo()
+        check(self.lib.owl_server_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in ServeInfo._fields_}
+
+    def now(self):
+        t = C.c_double()
+        check(self.lib.owl_server_now(self.h, C.byref(t)))
+        return t.value
+
+    def drain(self, max_rounds=1_000_000):
+        """Step until nothing is waiting or active; returns every result."""
+        out = []
+        for _ in range(max_rounds):
+            i = self.info()
+            if not i["waiting"] and not i["active"]:
+                break
+            out += self.step()
+        return out
+
+    def close(self):
+        if self.h:
+            check(self.lib.owl_server_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -26,6 +26,7 @@ typedef struct {
   uint32_t maxExp;
   int longCarry;
   char spec[64];     /* reference FFT spec, e.g. "1K:15:256:202" */
+  char rowKernel[32], colKernel[32];   /* the row / column pass kernels in use */
 } owl_gpu_info_t;
 
 typedef struct {
@@ -45,8 +46,11 @@ typedef struct {
 /* FFTConfig::bestFit (src/FFTConfig.cpp:235-263), without a tune file. spec may be "" or NULL. */
 int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen, uint32_t* N, uint32_t* maxExp);
 
-/* Gpu::Gpu / Gpu::make (src/Gpu.cpp:380-564). lineBlock 0 = automatic. */
-int owl_gpu_create(uint32_t E, const char* spec, int longCarry, uint32_t lineBlock, owl_gpu* gpu);
+/* Gpu::Gpu / Gpu::make (src/Gpu.cpp:380-564). lineBlock 0 = automatic.
+ * flags: OWL_LONG_CARRY = full carry every iteration (the reference's -carry long);
+ *        OWL_GENERIC = runtime-plan kernels even where a shape-specialized pass exists. */
+enum { OWL_LONG_CARRY = 1, OWL_GENERIC = 2 };
+int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, owl_gpu* gpu);
 int owl_gpu_destroy(owl_gpu gpu);
 int owl_gpu_info(owl_gpu gpu, owl_gpu_info_t* info);
 
@@ -79,6 +83,9 @@ int owl_gpu_read_roe(owl_gpu gpu, uint32_t* n, double* maxRoe, double* mean, dou
  * resStep iterations goes to res[0..resCap). */
 int owl_gpu_prp(owl_gpu gpu, uint32_t iters, uint32_t blockSize, uint32_t checkStep, uint64_t* res, uint32_t resStep,
                 uint32_t resCap, owl_prp_result_t* out);
+/* Failure-handling tests (the reference tests its recovery by hand, src/Gpu.cpp:1561-1575):
+   the next PRP run flips one data bit right after iteration k, once. 0 disarms. */
+int owl_gpu_arm_fault(owl_gpu gpu, uint32_t k);
 /* Gpu::isPrimeLL (src/Gpu.cpp:1586-1663) without savefiles: iters LL iterations from x0 = 4. */
 int owl_gpu_ll(owl_gpu gpu, uint32_t iters, uint64_t* res, uint32_t resStep, uint32_t resCap, owl_ll_result_t* out);
 
@@ -90,6 +97,61 @@ int owl_gpu_time_squares(owl_gpu gpu, uint32_t iters, uint32_t warmup, int useGr
 /* Per-kernel average durations over iters steady-state iterations, timed with events on the engine's
  * queue around each launch (no reference counterpart; the reference's -time profile is src/Gpu.cpp:773-799). */
 int owl_gpu_time_kernels(owl_gpu gpu, uint32_t iters, double* usCol, double* usRow);
+
+/* Device-to-device copy of the FP64 state (8*N bytes read, 8*N written: the same bytes as one
+ * pass), timed with events over iters copies: the streaming ceiling of the box and the
+ * calibration kernel for HBM traffic counters (no reference counterpart). */
+int owl_gpu_time_copy(owl_gpu gpu, uint32_t iters, double* usPerCopy);
+
+/* Batches of exponents (no reference counterpart: the reference runs one exponent per Gpu,
+ * with at most four worker threads on one device, src/main.cpp:110-117). Jobs of equal FFT
+ * geometry share every launch; groups of different geometry run on their own queues.
+ * spec "" gives every exponent the reference's FFT choice. res is n rows of resCap res64. */
+typedef struct owl_batch_s* owl_batch;
+int owl_batch_create(const uint32_t* exponents, uint32_t n, const char* spec, int flags, owl_batch* batch);
+int owl_batch_destroy(owl_batch batch);
+int owl_batch_info(owl_batch batch, uint32_t* nJobs, uint32_t* nGroups, uint64_t* totalWords);
+int owl_batch_job_info(owl_batch batch, uint32_t job, owl_gpu_info_t* info);
+int owl_batch_prp(owl_batch batch, uint32_t iters, uint32_t blockSize, uint32_t checkStep, uint64_t* res,
+                  uint32_t resStep, uint32_t resCap, owl_prp_result_t* out);
+int owl_batch_arm_fault(owl_batch batch, uint32_t job, uint32_t k);
+int owl_batch_ll(owl_batch batch, uint32_t iters, uint64_t* res, uint32_t resStep, uint32_t resCap,
+                 owl_ll_result_t* out);
+/* microseconds per batch iteration (every job squared once), optionally from captured graphs */
+int owl_batch_time_squares(owl_batch batch, uint32_t iters, uint32_t warmup, int useGraph, double* usPerIter);
+
+/* Serving (no reference counterpart: the reference has no request loop; each job follows
+ * Gpu::isPrimePRP / isPrimeLL, src/Gpu.cpp:1400-1663). Requests are admitted while their
+ * device footprint fits memBudget (0 = 3/4 of free memory) and fewer than maxJobs run; every
+ * owl_server_step admits, runs one round of blockSize iterations on every active job (jobs of
+ * one FFT geometry and kind share launches), verifies PRP jobs due for a Gerbicz check
+ * (every checkStep, and past the end) and retires finished ones. */
+typedef struct owl_server_s* owl_server;
+typedef struct {
+  uint32_t id, E, iters;
+  int ll;
+  uint64_t res64;   /* res64 of x_iters */
+  int isPrime;      /* iters == E (PRP, x == 9) or E - 2 (LL, x == 0) */
+  uint32_t nErrors, nChecks;
+  double maxRoe;
+  double tSubmit, tAdmit, tDone; /* seconds on the server's clock (owl_server_now) */
+  int status;       /* 0 done, 1 failed (repeated Gerbicz errors), 2 rejected */
+} owl_serve_result_t;
+typedef struct {
+  uint32_t waiting, active, groups, done, rounds;
+  uint64_t bytesInUse, budget;
+  uint64_t iterations;      /* requested squarings completed, all jobs */
+  uint64_t wordIterations;  /* the same weighted by FFT size N */
+} owl_serve_info_t;
+int owl_server_create(uint64_t memBudget, uint32_t maxJobs, uint32_t blockSize, uint32_t checkStep, const char* spec,
+                      int flags, owl_server* server);
+int owl_server_destroy(owl_server server);
+int owl_server_submit(owl_server server, uint32_t E, int ll, uint32_t iters, uint32_t* id);
+int owl_server_step(owl_server server, uint32_t* nDone);
+/* up to cap finished results (oldest first); *n is how many were written */
+int owl_server_take(owl_server server, owl_serve_result_t* out, uint32_t cap, uint32_t* n);
+int owl_server_info(owl_server server, owl_serve_info_t* info);
+int owl_server_now(owl_server server, double* secs);
 
 /* Kernel-level access for numerics tests (no reference counterpart; the reference's
  * closest tools are its -verbose self-tests, src/Gpu.cpp:1059-1093).

@@ -10,25 +10,27 @@ from tests.fftmodel import Model
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [("64:2:64", 12.0), ("256:2:256", 15.0), ("256:3:256", 8.0), ("1K:3:256", 8.0), ("256:13:256", 8.0),
-          ("512:9:256", 8.0)]
+# (spec, bits per word, force the runtime-plan kernels)
+SHAPES = [("64:2:64", 12.0, False), ("256:2:256", 15.0, False), ("256:2:256", 15.0, True), ("256:3:256", 8.0, False),
+          ("1K:3:256", 8.0, False), ("1K:3:256", 8.0, True), ("256:13:256", 8.0, False), ("512:9:256", 8.0, False),
+          ("1K:15:256", 17.6, False), ("1K:15:256", 17.6, True)]
 
 
 def rel(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-def setup(spec, bpw, seed=1):
+def setup(spec, bpw, generic, seed=1):
     N = owl.best_fit(1000, spec)[1]
     E = workloads.next_prime(int(bpw * N))
-    g = owl.Gpu(E, spec)
+    g = owl.Gpu(E, spec, generic=generic)
     m = Model(E, g.W, g.H, g.R)
     return g, m, np.random.default_rng(seed)
 
 
-@pytest.mark.parametrize("spec,bpw", SHAPES)
-def test_lead_in_and_square(spec, bpw):
-    g, m, rng = setup(spec, bpw)
+@pytest.mark.parametrize("spec,bpw,generic", SHAPES)
+def test_lead_in_and_square(spec, bpw, generic):
+    g, m, rng = setup(spec, bpw, generic)
     with g:
         d = m.random_digits(rng)
         g.write_digits(owl.DATA, d.tolist())
@@ -40,9 +42,9 @@ def test_lead_in_and_square(spec, bpw):
         assert rel(sq, m.col_square(f)) < 1e-12
 
 
-@pytest.mark.parametrize("spec,bpw", SHAPES)
-def test_fused_row_pass(spec, bpw):
-    g, m, rng = setup(spec, bpw, seed=2)
+@pytest.mark.parametrize("spec,bpw,generic", SHAPES)
+def test_fused_row_pass(spec, bpw, generic):
+    g, m, rng = setup(spec, bpw, generic, seed=2)
     with g:
         d = m.random_digits(rng)
         g.write_digits(owl.DATA, d.tolist())
@@ -58,9 +60,9 @@ def test_fused_row_pass(spec, bpw):
         assert len(bad) == 0, f"{len(bad)} elements differ; first at HBM index {bad[:8]}"
 
 
-@pytest.mark.parametrize("spec,bpw", SHAPES)
-def test_carry_out_digits(spec, bpw):
-    g, m, rng = setup(spec, bpw, seed=3)
+@pytest.mark.parametrize("spec,bpw,generic", SHAPES)
+def test_carry_out_digits(spec, bpw, generic):
+    g, m, rng = setup(spec, bpw, generic, seed=3)
     with g:
         d = m.random_digits(rng)
         g.write_digits(owl.DATA, d.tolist())
@@ -72,3 +74,33 @@ def test_carry_out_digits(spec, bpw):
         want, roe = m.full_digits(m.row_inverse(sq))
         bad = np.nonzero(got != want)[0]
         assert len(bad) == 0, f"{len(bad)} digits differ; first at k = {bad[:8]}"
+
+
+def test_kernel_selection():
+    # the 7.5M-word shape runs the shape-specialized passes; generic=True forces the runtime-plan ones
+    E = workloads.next_prime(139_000_000)
+    with owl.Gpu(E) as g:
+        assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840")
+    with owl.Gpu(E, generic=True) as g:
+        assert g.row_kernel.startswith("row") and not g.row_kernel.startswith("rowx")
+        assert g.col_kernel.startswith("col") and not g.col_kernel.startswith("colx")
+
+
+@pytest.mark.parametrize("spec,bpw", [("1K:3:256", 12.0), ("1K:15:256", 17.6)])
+def test_mul_pass(spec, bpw):
+    # COL_FWD on a second operand, then COL_MUL, against the model's spectrum product
+    g, m, rng = setup(spec, bpw, False, seed=4)
+    with g:
+        a, b = m.random_digits(rng), m.random_digits(rng)
+        g.write_digits(owl.AUX, b.tolist())
+        g.write_digits(owl.DATA, a.tolist())
+        g.run_pass(g.PASS_LEADIN, owl.AUX)
+        fb = g.read_freq()
+        g.write_freq(fb, 2)
+        g.run_pass(g.PASS_SPECTRUM)
+        spec_gpu = g.read_freq(2)
+        assert rel(spec_gpu, m.spectrum_layout(fb)) < 1e-12
+        g.run_pass(g.PASS_LEADIN, owl.DATA)
+        fa = g.read_freq()
+        g.run_pass(g.PASS_MUL)
+        assert rel(g.read_freq(), m.col_square(fa, mul_spec=m.spectrum(fb))) < 1e-12

@@ -58,12 +58,14 @@ def test_ll_res64_vs_gmp(spec, bpw):
         assert out.res64 == res[-1]
 
 
-@pytest.mark.parametrize("spec,bpw", [("64:2:64", 12.0), ("256:2:256", 15.0), ("256:13:256", 8.0)])
-def test_full_residue_vs_gmp(spec, bpw):
+@pytest.mark.parametrize("spec,bpw,generic", [("64:2:64", 12.0, False), ("256:2:256", 15.0, False),
+                                              ("256:2:256", 15.0, True), ("256:13:256", 8.0, False),
+                                              ("1K:3:256", 12.0, False), ("1K:3:256", 12.0, True)])
+def test_full_residue_vs_gmp(spec, bpw, generic):
     E, N = exponent_for(spec, bpw)
-    announce("residue", spec, E)
+    announce("residue", spec, E, "generic" if generic else "")
     want = oracle.gmp_residue(E, 50)
-    with owl.Gpu(E, spec) as g:
+    with owl.Gpu(E, spec, generic=generic) as g:
         g.write_int(owl.DATA, 3)
         g.square_loop(owl.DATA, owl.DATA, 0, 50)
         same = g.read_words(owl.DATA) == want   # (not inside assert: pytest would diff megabit lists)
@@ -120,6 +122,19 @@ def test_gerbicz_check_and_fault():
         assert not g.do_check(B)
 
 
+def test_prp_recovers_from_fault_after_a_verified_check():
+    # one flipped data bit at k = 230: the check at 300 fails, the run reloads the state
+    # verified at 200 (whose check already holds x_200) and ends on GMP's residues
+    spec = "256:2:256"
+    E = workloads.primes_from(3_800_000, 1)[0]
+    announce("gerbicz recovery", spec, E)
+    with owl.Gpu(E, spec) as g:
+        g.arm_fault(230)
+        out, rows = g.prp(400, block_size=50, check_step=100, res_step=100)
+    assert out.nErrors == 1 and out.nChecks == 4
+    assert rows == oracle.gmp_res64(E, 400, 100)
+
+
 def test_is_equal_and_copy():
     spec = "64:2:64"
     E, N = exponent_for(spec, 12.0)
@@ -142,11 +157,12 @@ def test_config_a_golden(golden):
         assert out.nErrors == 0 and out.maxRoe < 0.4
 
 
-def test_config_b_golden_short(golden):
+@pytest.mark.parametrize("generic", [False, True])
+def test_config_b_golden_short(golden, generic):
     head, res = golden("B_prp_short")
     E = head["E"]
-    announce("config B prp", E)
-    with owl.Gpu(E) as g:
+    announce("config B prp", E, "generic" if generic else "")
+    with owl.Gpu(E, generic=generic) as g:
         assert g.spec == "1K:15:256:101" and g.N == 7_864_320
         out, got = g.prp(200, block_size=100, check_step=100, res_step=100)
         assert got == [res[100], res[200]]

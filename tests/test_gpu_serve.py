@@ -1,0 +1,88 @@
+"""Serving: requests admitted while others run (continuous batching) end on the same
+res64 as the same request run alone and as GMP; admission respects the memory budget."""
+import sys
+
+import pytest
+
+import gpuowl_amd as owl
+import oracle
+from gpuowl_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(*a):
+    print("CASE", *a, file=sys.stderr, flush=True)
+
+
+def expected(E, kind, iters):
+    return oracle.gmp_res64(E, iters, iters, ll=(kind == "ll"))[0]
+
+
+def alone(E, kind, iters):
+    with owl.Gpu(E) as g:
+        if kind == "ll":
+            out, _ = g.ll(iters)
+        else:
+            out, _ = g.prp(iters, block_size=200, check_step=400)
+        return out.res64
+
+
+def test_mix_matches_alone_and_gmp():
+    # two FFT geometries (256K and 384K words), PRP and LL, lengths off the round size,
+    # requests arriving while the first ones run
+    first = [(2_000_003, "prp", 1234), (2_000_029, "ll", 1000), (5_300_023, "prp", 801)]
+    later = [(2_000_081, "prp", 1399), (5_300_039, "ll", 617), (3_900_067, "prp", 400)]
+    announce("serve mix", first + later)
+    got = {}
+    with owl.Server(block_size=200, check_step=400) as s:
+        ids = {s.submit(E, k, n): (E, k, n) for E, k, n in first}
+        for r in s.step() + s.step():
+            got[r["id"]] = r
+        assert s.info()["active"] == 3
+        ids.update({s.submit(E, k, n): (E, k, n) for E, k, n in later})
+        for r in s.drain():
+            got[r["id"]] = r
+        info = s.info()
+    assert set(got) == set(ids)
+    assert info["iterations"] == sum(n for _, _, n in first + later)
+    for rid, (E, k, n) in ids.items():
+        r = got[rid]
+        assert r["status"] == 0 and r["nErrors"] == 0 and r["maxRoe"] < 0.4, r
+        assert r["res64"] == expected(E, k, n), (E, k, n)
+        assert r["res64"] == alone(E, k, n), (E, k, n)
+        if k == "prp":
+            assert r["nChecks"] >= 1
+        assert r["tSubmit"] <= r["tAdmit"] <= r["tDone"]
+
+
+def test_admission_by_memory_budget():
+    exps = workloads.primes_from(3_000_000, 4)
+    spec = "256:2:256"
+    N = owl.best_fit(exps[0], spec)[1]
+    one = 120 * N  # above one job's footprint bound, below two
+    announce("serve budget", exps)
+    with owl.Server(mem_budget=2 * one, block_size=100, check_step=200, spec=spec) as s:
+        for E in exps:
+            s.submit(E, "prp", 300)
+        s.step()
+        i = s.info()
+        assert i["active"] == 2 and i["waiting"] == 2 and i["bytesInUse"] <= i["budget"]
+        res = s.drain()
+    assert sorted(r["E"] for r in res) == exps
+    for r in res:
+        assert r["res64"] == expected(r["E"], "prp", 300)
+
+
+@pytest.mark.slow
+def test_full_mersenne_prime_runs_to_completion():
+    # M859433 is prime: PRP ends on 9 at k = E and LL on 0 at k = E - 2 (long-carry geometry)
+    E = 859_433
+    announce("serve full", E)
+    with owl.Server(block_size=1000, check_step=50_000) as s:
+        s.submit(E, "prp")
+        s.submit(E, "ll")
+        res = s.drain()
+    assert len(res) == 2
+    for r in res:
+        assert r["status"] == 0 and r["isPrime"], r

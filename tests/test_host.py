@@ -78,3 +78,20 @@ def test_workloads():
     assert workloads.config_b() == 139_000_003
     c = workloads.config_c()
     assert len(c) == 64 and c[0] == 20_000_003 and c[-1] == 20_000_879
+
+
+def test_server_rejects_bad_requests_before_running():
+    # no device work happens before the first step, so this runs without a GPU
+    with gpuowl_amd.Server(mem_budget=1 << 30) as s:
+        with pytest.raises(gpuowl_amd.OwlError):
+            s.submit(4_000_037, "prp", 0)
+        with pytest.raises(gpuowl_amd.OwlError):
+            s.submit(4_000_037, "prp", 4_000_038)
+        with pytest.raises(gpuowl_amd.OwlError):
+            s.submit(4_000_037, "ll", 4_000_036)
+        with pytest.raises(gpuowl_amd.OwlError):
+            s.submit(756_839, "prp", 10)   # below 3 bits per word at the smallest FFT
+        rid = s.submit(4_000_037, "ll", 4_000_035)
+        assert rid == 1 and s.info()["waiting"] == 1
+    with pytest.raises(gpuowl_amd.OwlError):
+        gpuowl_amd.Server(mem_budget=1 << 30, block_size=100, check_step=150)
