@@ -94,6 +94,56 @@ def batch_leg(owl, exps, spec, iters, warmup):
             "hbm_frac_32N_model": round(32 * rate / (HBM_PEAK_GBS * 1e9), 4)}
 
 
+def pmc_pass(counter, steps=20, warmup=5):
+    """One rocprofv3 counter pass over `bench.py --profile` in a child process.
+    Returns {(kernel name, grid size): [values per dispatch]}."""
+    import csv
+    import glob
+    import shutil
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    out = tempfile.mkdtemp(prefix="owl_pmc_", dir="/tmp")
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+           sys.executable, os.path.join(ROOT, "bench.py"), "--profile", "--steps", str(steps), "--warmup", str(warmup)]
+    try:
+        subprocess.run(cmd, capture_output=True, text=True, timeout=90, cwd="/tmp",
+                       env=dict(os.environ, TMPDIR="/tmp"))
+    except subprocess.TimeoutExpired:
+        return None
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        return None
+    rows = {}
+    for r in csv.DictReader(open(files[0])):
+        if r["Counter_Name"] == counter:
+            rows.setdefault((r["Kernel_Name"], int(r["Grid_Size"])), []).append(float(r["Counter_Value"]))
+    shutil.rmtree(out, ignore_errors=True)
+    return rows
+
+
+def pmc_traffic(kernel_prefix, copy_bytes):
+    """HBM bytes per launch of the kernel whose name starts with kernel_prefix (its most
+    frequent grid), from FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
+    for the state copy of copy_bytes read + copy_bytes written in the same run."""
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        rows = pmc_pass(counter)
+        if not rows:
+            return None
+        copy = [v for (k, g), vs in rows.items() if "copyBuffer" in k for v in vs if v * 1024 > copy_bytes / 4]
+        mine = {(k, g): vs for (k, g), vs in rows.items() if k.split("(")[0].split("<")[0].endswith(kernel_prefix)}
+        if not copy or not mine:
+            return None
+        key = max(mine, key=lambda kg: len(mine[kg]))
+        factor = copy_bytes / (sum(copy) / len(copy) * 1024)
+        got[counter] = {"kib": round(sum(mine[key]) / len(mine[key]), 1), "factor": round(factor, 4),
+                        "kernel": key[0].split("(")[0], "grid": key[1]}
+    total = sum(v["kib"] * 1024 * v["factor"] for v in got.values())
+    return {"bytes": round(total), "fetch": got["FETCH_SIZE"], "write": got["WRITE_SIZE"]}
+
+
 def percentile(xs, q):
     xs = sorted(xs)
     if not xs:
@@ -144,6 +194,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched legs (configs C and D)")
     ap.add_argument("--batch-iters", type=int, default=100)
@@ -194,28 +245,50 @@ def main():
     us_copy = g.time_copy(50)
     roe = g.read_roe()
 
-    alg_bytes = 16 * N  # per launch: the FP64 state (8 B per word) read once and written once
+    # Algorithmic bytes: 64*N per iteration, 16*N (state read once, written once) for each of
+    # the reference's four passes. The row pass does one of them (carryFused), the column pass
+    # three (middle-in, tail square, middle-out).
+    alg = {"row": 16 * N, "col": 48 * N}
     kernels = {"col": us_col, "row": us_row}
+    names = {"row": g.row_kernel, "col": g.col_kernel}
     dom = max(kernels, key=kernels.get)
-    achieved = alg_bytes / (kernels[dom] * 1e-6) / 1e9
+    achieved = alg[dom] / (kernels[dom] * 1e-6) / 1e9
+    copy_bytes = 8 * N  # the state copy reads 8*N bytes and writes 8*N bytes
+    traffic = None
+    if not a.no_pmc:
+        prefix = "k_rowx" if names[dom].startswith("rowx") else "k_colx" if names[dom].startswith("colx") else \
+            "k_row" if dom == "row" else "k_col"
+        traffic = pmc_traffic(prefix, copy_bytes)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_col (column pass: pairing + square)" if dom == "col" else "k_row (row pass: fused carry)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["bytes"] if traffic else None,
+                "kernel": f"{names[dom]} ({'row pass: fused carry' if dom == 'row' else 'column pass: middle + square'})",
+                "alg_bytes_per_launch": alg[dom],
                 "us_per_launch": {"col": round(us_col, 2), "row": round(us_row, 2)},
-                "alg_bytes_per_launch": alg_bytes,
-                "copy_gbs": round(alg_bytes / (us_copy * 1e-6) / 1e9, 1)}
+                "traffic_detail": traffic,
+                "iteration": {"alg_bytes": 64 * N, "us": round(us_col + us_row, 2),
+                              "frac": round(64 * N / ((us_col + us_row) * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
+                "copy_gbs": round(2 * copy_bytes / (us_copy * 1e-6) / 1e9, 1)}
     g.close()
+
+    def leg(fn, *args):
+        # an extra leg that fails is reported, never allowed to drop the bench line
+        try:
+            return fn(*args)
+        except Exception as e:  # noqa: BLE001
+            return {"error": f"{type(e).__name__}: {e}"}
 
     batched = None
     if not a.no_batch:
         batched = {
-            "C_uniform_64x1K:3:256": batch_leg(owl, workloads.config_c(), "1K:3:256", a.batch_iters, 10),
-            "D_uneven_200": batch_leg(owl, workloads.config_d(), "", a.batch_iters, 10),
+            "C_uniform_64x1K:3:256": leg(batch_leg, owl, workloads.config_c(), "1K:3:256", a.batch_iters, 10),
+            "D_uneven_200": leg(batch_leg, owl, workloads.config_d(), "", a.batch_iters, 10),
         }
 
     served = None
     if not a.no_serve:
-        served = serve_leg(owl, workloads, a.serve_requests, a.serve_lambda, a.serve_slice, a.serve_full_below)
+        served = leg(serve_leg, owl, workloads, a.serve_requests, a.serve_lambda, a.serve_slice,
+                     a.serve_full_below)
 
     cpu = None
     if not a.no_cpu:

@@ -50,6 +50,14 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
   N = fft.size();
   ND = N / 2;
   if (W * H != ND) { throw std::invalid_argument("bad FFT geometry"); }
+  // The column pass holds a column pair in LDS (H <= 4096). Taller reference shapes
+  // (e.g. 256:9:512, H = 4608) are factored again as W' x H' with W' a larger power
+  // of two: the FFT length N, which fixes the words, weights and results, stays the
+  // reference's; only the split of the ND-point FFT between the passes changes.
+  if (H > 4096) {
+    while (W < 1024 && H % 2 == 0 && H / 2 >= 256) { W *= 2, H /= 2; }
+    while (H > 4096 && H % 2 == 0 && W < 4096) { W *= 2, H /= 2; }
+  }
   if (H > 4096 || W > 4096 || H < 64) { throw std::invalid_argument("FFT shape " + fft.spec() + " not supported"); }
   double bpw = double(E) / N;
   if (bpw < FFTShape::MIN_BPW) { throw std::invalid_argument("FFT size too large for exponent"); }

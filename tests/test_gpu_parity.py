@@ -33,6 +33,8 @@ SHAPES = [
     ("512:9:256", 8.0),
     ("1K:3:256", 8.0),
     ("512:6:512", 5.0),
+    ("256:9:512", 12.0),    # H = 4608: run as 1024 x 1152
+    ("256:15:512", 17.0),   # H = 7680: run as 1024 x 1920
 ]
 
 

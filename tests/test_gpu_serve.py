@@ -60,7 +60,7 @@ def test_admission_by_memory_budget():
     exps = workloads.primes_from(3_000_000, 4)
     spec = "256:2:256"
     N = owl.best_fit(exps[0], spec)[1]
-    one = 120 * N  # above one job's footprint bound, below two
+    one = 60 * N  # a job's footprint bound is about 52 bytes per word
     announce("serve budget", exps)
     with owl.Server(mem_budget=2 * one, block_size=100, check_step=200, spec=spec) as s:
         for E in exps:
