@@ -42,6 +42,12 @@ class ServeInfo(C.Structure):
                 ("iterations", C.c_uint64), ("wordIterations", C.c_uint64)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("lineBlock", C.c_uint32), ("flags", C.c_int), ("usPerIter", C.c_double),
+                ("W", C.c_uint32), ("H", C.c_uint32), ("R", C.c_uint32), ("longCarry", C.c_int),
+                ("rowKernel", C.c_char * 32), ("colKernel", C.c_char * 32)]
+
+
 u32, i32, u64, dbl, vp, cp = C.c_uint32, C.c_int, C.c_uint64, C.c_double, C.c_void_p, C.c_char_p
 P = C.POINTER
 
@@ -102,6 +108,12 @@ SIGNATURES = {
     "owl_batch_arm_fault": [vp, u32, u32],
     "owl_batch_ll": [vp, u32, P(u64), u32, u32, P(LLResult)],
     "owl_batch_time_squares": [vp, u32, u32, i32, P(dbl)],
+    "owl_tune_run": [u32, cp, u32, P(TuneEntry), u32, P(u32)],
+    "owl_tune_set": [u32, u32, u32, i32, dbl],
+    "owl_tune_get": [u32, P(u32), P(u32), P(i32), P(dbl), P(i32)],
+    "owl_tune_clear": [],
+    "owl_tune_load": [cp],
+    "owl_tune_save": [cp],
     "owl_server_create": [u64, u32, u32, u32, cp, i32, P(vp)],
     "owl_server_destroy": [vp],
     "owl_server_submit": [vp, u32, i32, u32, P(u32)],

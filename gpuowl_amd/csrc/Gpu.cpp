@@ -3,6 +3,7 @@
 
 #include "Trig.h"
 #include "TrigBufCache.h"
+#include "Tune.h"
 
 #include <algorithm>
 #include <chrono>
@@ -43,18 +44,32 @@ bool equals9(const Words& w) {
   return std::all_of(w.begin() + 1, w.end(), [](u32 x) { return x == 0; });
 }
 
-Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn} {
+Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fftIn} {
+  GpuOptions opt = optIn;
   owlCheck(owl_hip_set_device(opt.device), "set device");
   W = fft.shape.width;
   H = fft.shape.middle * fft.shape.height;
   N = fft.size();
   ND = N / 2;
   if (W * H != ND) { throw std::invalid_argument("bad FFT geometry"); }
-  // The column pass holds a column pair in LDS (H <= 4096). Taller reference shapes
-  // (e.g. 256:9:512, H = 4608) are factored again as W' x H' with W' a larger power
-  // of two: the FFT length N, which fixes the words, weights and results, stays the
-  // reference's; only the split of the ND-point FFT between the passes changes.
-  if (H > 4096) {
+  // a tuned choice for this FFT length applies when the caller chose nothing
+  TuneChoice tuned;
+  if (!opt.width && !opt.lineBlock && !opt.generic && !opt.colTwoRegion && TuneTable::get(N, &tuned)) {
+    opt.width = tuned.width;
+    opt.lineBlock = tuned.lineBlock;
+    opt.generic = tuned.generic;
+    opt.colTwoRegion = tuned.colTwoRegion;
+    tunedLineBlock_ = true;
+  }
+  if (opt.width) {
+    if (ND % opt.width) { throw std::invalid_argument("width does not divide the FFT length"); }
+    W = opt.width;
+    H = ND / W;
+  } else if (H > 4096) {
+    // The column pass holds a column pair in LDS (H <= 4096). Taller reference shapes
+    // (e.g. 256:9:512, H = 4608) are factored again as W' x H' with W' a larger power
+    // of two: the FFT length N, which fixes the words, weights and results, stays the
+    // reference's; only the split of the ND-point FFT between the passes changes.
     while (W < 1024 && H % 2 == 0 && H / 2 >= 256) { W *= 2, H /= 2; }
     while (H > 4096 && H % 2 == 0 && W < 4096) { W *= 2, H /= 2; }
   }
@@ -72,6 +87,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
   const u32 rMax = std::min<u32>(4096 / W, 16);
   auto fusedStable = [&](u32 r) { return (2.0 * r - 1) * (E / N) >= 0.5 * std::log2(double(N)) + 4; };
   R = opt.lineBlock;
+  if (R && tunedLineBlock_ && (!fusedStable(R) || H % R || R * W > 4096)) { R = 0; }  // tuned for another exponent
   if (!R) {
     R = rMax;
     while (R > 1 && H / R < 256) { R /= 2; }
@@ -87,9 +103,14 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
     return owl_hip_kernel_get(name.c_str(), &k) == 0;
   };
   rowKernel = "rowx" + std::to_string(W) + "r" + std::to_string(R);
-  colKernel = "colx" + std::to_string(H);
   bool rowFixed = !opt.generic && haveKernel(rowKernel);
-  bool colFixed = !opt.generic && haveKernel(colKernel) && (H / 16) % R == 0;
+  // column pass: k_coly (one LDS region, pairing in registers) unless the two-region
+  // k_colx is asked for; k_colx's lane mapping needs (H/16) % R == 0
+  const std::string colY = "coly" + std::to_string(H), colX = "colx" + std::to_string(H);
+  bool useY = !opt.generic && !opt.colTwoRegion && haveKernel(colY);
+  bool useX = !opt.generic && !useY && haveKernel(colX) && (H / 16) % R == 0;
+  colKernel = useY ? colY : colX;
+  bool colFixed = useY || useX;
 
   rowPlan = fftPlan(W);
   colPlan = fftPlan(H);
@@ -109,7 +130,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &rowT), "kernel");
-    rowLds = R * (W + W / 16) * 16;
+    rowLds = R * (W + W / 16 + 4) * 16;  // rowRegion<W>() in kfixed.hip
   } else {
     rowKernel = "row" + std::to_string(rowT);
   }
@@ -117,7 +138,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& opt) : E{E}, fft{fftIn
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
-    colLds = 2 * (H + H / 16) * 16;
+    colLds = (useY ? 1 : 2) * (H + H / 16 + 8) * 16;  // colRegion<H>() in kfixed.hip
   } else {
     colKernel = "col" + std::to_string(colT);
   }

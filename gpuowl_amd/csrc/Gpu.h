@@ -38,6 +38,8 @@ struct RoeInfo {
 struct GpuOptions {
   bool longCarry = false;  // lead-out + lead-in around every iteration (the reference's -carry long)
   bool generic = false;    // runtime-plan kernels even where a shape-specialized pass exists
+  bool colTwoRegion = false;  // the two-region column pass (k_colx) where k_coly exists
+  u32 width = 0;           // W of the W x H split of the FFT; 0 = the reference's (refactored when too tall)
   u32 lineBlock = 0;       // lines per row-pass block; 0 = automatic
   int device = 0;
 };
@@ -183,6 +185,7 @@ private:
   u32 roePos = 0;
   u32 stairTag_ = 0, lastTag_ = 0, resLo_ = 1, resHi_ = 0;
   u32 faultAt_ = 0;
+  bool tunedLineBlock_ = false;
   Kernel kRow, kCol, kCarryB, kCarryFix, kWordsIn, kWordsOut, kResidue, kIsEqual, kSum64, kCopy;
 
   template<typename T> T* alloc(size_t n);

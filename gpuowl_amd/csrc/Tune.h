@@ -1,0 +1,39 @@
+// Autotuning of the engine's FFT split, line block and pass variants per FFT length
+// (in place of the reference's tune.cpp / tune.txt, src/tune.cpp:328-738,
+// src/TuneEntry.cpp:7-84, which tune the choice of FFT *length* per exponent; here the
+// length stays FFTConfig::bestFit's and the tuner picks how the engine runs it).
+#pragma once
+
+#include "Gpu.h"
+
+#include <string>
+#include <vector>
+
+struct TuneChoice {
+  u32 width = 0;      // W of the W x H split (0 = the engine's rule)
+  u32 lineBlock = 0;  // R (0 = the engine's rule)
+  bool generic = false, colTwoRegion = false;
+  double us = 0;      // measured microseconds per iteration
+};
+
+// Process-wide table N -> choice, consulted by Gpu when no option overrides it.
+namespace TuneTable {
+void set(u32 N, const TuneChoice& c);
+bool get(u32 N, TuneChoice* c);
+void clear();
+// text lines "N width lineBlock flags us" (flags: 2 = generic, 4 = two-region column pass)
+void load(const std::string& path);
+void save(const std::string& path);
+}  // namespace TuneTable
+
+struct TuneResult {
+  TuneChoice choice;
+  u32 W = 0, H = 0, R = 0;
+  bool longCarry = false;
+  std::string rowKernel, colKernel;
+};
+
+// Time every candidate (split, line block, pass variant) for exponent E at the FFT
+// length bestFit(E, spec) gives: steady squarings replayed from graphs, iters of them.
+// Fastest first. Candidates that the engine rejects are skipped.
+std::vector<TuneResult> tuneShape(u32 E, const std::string& spec, u32 iters, u32 warmup, int device = 0);

@@ -4,9 +4,11 @@
 #include "Gpu.h"
 #include "GpuBatch.h"
 #include "Server.h"
+#include "Tune.h"
 
 #include <deque>
 
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -67,6 +69,7 @@ int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, 
     GpuOptions opt;
     opt.longCarry = flags & OWL_LONG_CARRY;
     opt.generic = flags & OWL_GENERIC;
+    opt.colTwoRegion = flags & OWL_COL_TWO_REGION;
     opt.lineBlock = lineBlock;
     *out = new owl_gpu_s{new Gpu(E, fft, opt)};
   });
@@ -267,6 +270,61 @@ int owl_gpu_time_copy(owl_gpu g, uint32_t iters, double* us) {
   });
 }
 
+int owl_tune_run(uint32_t E, const char* spec, uint32_t iters, owl_tune_entry_t* out, uint32_t cap, uint32_t* n) {
+  return guarded([&] {
+    std::vector<TuneResult> rs = tuneShape(E, spec ? spec : "", iters, std::max<u32>(10, iters / 10));
+    u32 i = 0;
+    for (; i < cap && i < rs.size(); ++i) {
+      const TuneResult& r = rs[i];
+      owl_tune_entry_t& o = out[i];
+      o.width = r.choice.width;
+      o.lineBlock = r.choice.lineBlock;
+      o.flags = (r.choice.generic ? OWL_GENERIC : 0) | (r.choice.colTwoRegion ? OWL_COL_TWO_REGION : 0);
+      o.usPerIter = r.choice.us;
+      o.W = r.W, o.H = r.H, o.R = r.R;
+      o.longCarry = r.longCarry;
+      std::snprintf(o.rowKernel, sizeof o.rowKernel, "%s", r.rowKernel.c_str());
+      std::snprintf(o.colKernel, sizeof o.colKernel, "%s", r.colKernel.c_str());
+    }
+    *n = i;
+  });
+}
+
+int owl_tune_set(uint32_t N, uint32_t width, uint32_t lineBlock, int flags, double us) {
+  return guarded([&] {
+    TuneChoice c;
+    c.width = width;
+    c.lineBlock = lineBlock;
+    c.generic = flags & OWL_GENERIC;
+    c.colTwoRegion = flags & OWL_COL_TWO_REGION;
+    c.us = us;
+    TuneTable::set(N, c);
+  });
+}
+
+int owl_tune_get(uint32_t N, uint32_t* width, uint32_t* lineBlock, int* flags, double* us, int* found) {
+  return guarded([&] {
+    TuneChoice c;
+    *found = TuneTable::get(N, &c);
+    *width = c.width;
+    *lineBlock = c.lineBlock;
+    *flags = (c.generic ? OWL_GENERIC : 0) | (c.colTwoRegion ? OWL_COL_TWO_REGION : 0);
+    *us = c.us;
+  });
+}
+
+int owl_tune_clear(void) {
+  return guarded([&] { TuneTable::clear(); });
+}
+
+int owl_tune_load(const char* path) {
+  return guarded([&] { TuneTable::load(path); });
+}
+
+int owl_tune_save(const char* path) {
+  return guarded([&] { TuneTable::save(path); });
+}
+
 int owl_server_create(uint64_t memBudget, uint32_t maxJobs, uint32_t blockSize, uint32_t checkStep, const char* spec,
                       int flags, owl_server* out) {
   *out = nullptr;
@@ -279,6 +337,7 @@ int owl_server_create(uint64_t memBudget, uint32_t maxJobs, uint32_t blockSize, 
     o.spec = spec ? spec : "";
     o.gpu.longCarry = flags & OWL_LONG_CARRY;
     o.gpu.generic = flags & OWL_GENERIC;
+    o.gpu.colTwoRegion = flags & OWL_COL_TWO_REGION;
     *out = new owl_server_s{new Server(o), {}};
   });
 }
@@ -354,6 +413,7 @@ int owl_batch_create(const uint32_t* exponents, uint32_t n, const char* spec, in
     GpuOptions opt;
     opt.longCarry = flags & OWL_LONG_CARRY;
     opt.generic = flags & OWL_GENERIC;
+    opt.colTwoRegion = flags & OWL_COL_TWO_REGION;
     *out = new owl_batch_s{new GpuBatch(std::vector<u32>(exponents, exponents + n), spec ? spec : "", opt)};
   });
 }

@@ -2,7 +2,7 @@
 src/Gpu.h:213-300, src/FFTConfig.h:69-87) over the C-ABI in include/owl.h."""
 import ctypes as C
 
-from ._lib import GpuInfo, LLResult, OwlError, PrpResult, ServeInfo, ServeResult, check, load
+from ._lib import GpuInfo, LLResult, OwlError, PrpResult, ServeInfo, ServeResult, TuneEntry, check, load
 
 DATA, AUX, CHECK, BASE = 0, 1, 2, 3
 
@@ -29,12 +29,13 @@ def int_to_words(x, E):
 
 
 class Gpu:
-    """One exponent on the GPU: Gpu(E, spec="", long_carry=False, line_block=0, generic=False)."""
+    """One exponent on the GPU: Gpu(E, spec="", long_carry=False, line_block=0, generic=False,
+    col_two_region=False)."""
 
-    def __init__(self, E, spec="", long_carry=False, line_block=0, generic=False):
+    def __init__(self, E, spec="", long_carry=False, line_block=0, generic=False, col_two_region=False):
         self.lib = load()
         self.h = C.c_void_p()
-        flags = (1 if long_carry else 0) | (2 if generic else 0)
+        flags = (1 if long_carry else 0) | (2 if generic else 0) | (4 if col_two_region else 0)
         check(self.lib.owl_gpu_create(E, (spec or "").encode(), flags, line_block, C.byref(self.h)))
         info = GpuInfo()
         check(self.lib.owl_gpu_info(self.h, C.byref(info)))
@@ -318,3 +319,38 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def tune(E, spec="", iters=200):
+    """Time every engine configuration for exponent E at bestFit(E, spec); fastest first.
+    Each entry: width, line_block, flags, us, W, H, R, long_carry, row, col."""
+    lib = load()
+    buf = (TuneEntry * 128)()
+    n = C.c_uint32()
+    check(lib.owl_tune_run(E, (spec or "").encode(), iters, buf, 128, C.byref(n)))
+    return [{"width": e.width, "line_block": e.lineBlock, "flags": e.flags, "us": e.usPerIter, "W": e.W, "H": e.H,
+             "R": e.R, "long_carry": bool(e.longCarry), "row": e.rowKernel.decode(), "col": e.colKernel.decode()}
+            for e in buf[:n.value]]
+
+
+def tune_set(N, width, line_block, flags, us=0.0):
+    check(load().owl_tune_set(N, width, line_block, flags, us))
+
+
+def tune_get(N):
+    """The recorded choice for FFT length N, or None."""
+    w, r, f, us, found = C.c_uint32(), C.c_uint32(), C.c_int(), C.c_double(), C.c_int()
+    check(load().owl_tune_get(N, C.byref(w), C.byref(r), C.byref(f), C.byref(us), C.byref(found)))
+    return {"width": w.value, "line_block": r.value, "flags": f.value, "us": us.value} if found.value else None
+
+
+def tune_clear():
+    check(load().owl_tune_clear())
+
+
+def tune_load(path):
+    check(load().owl_tune_load(str(path).encode()))
+
+
+def tune_save(path):
+    check(load().owl_tune_save(str(path).encode()))

@@ -48,8 +48,9 @@ int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen,
 
 /* Gpu::Gpu / Gpu::make (src/Gpu.cpp:380-564). lineBlock 0 = automatic.
  * flags: OWL_LONG_CARRY = full carry every iteration (the reference's -carry long);
- *        OWL_GENERIC = runtime-plan kernels even where a shape-specialized pass exists. */
-enum { OWL_LONG_CARRY = 1, OWL_GENERIC = 2 };
+ *        OWL_GENERIC = runtime-plan kernels even where a shape-specialized pass exists;
+ *        OWL_COL_TWO_REGION = the column pass variant that keeps both columns in LDS. */
+enum { OWL_LONG_CARRY = 1, OWL_GENERIC = 2, OWL_COL_TWO_REGION = 4 };
 int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, owl_gpu* gpu);
 int owl_gpu_destroy(owl_gpu gpu);
 int owl_gpu_info(owl_gpu gpu, owl_gpu_info_t* info);
@@ -152,6 +153,29 @@ int owl_server_step(owl_server server, uint32_t* nDone);
 int owl_server_take(owl_server server, owl_serve_result_t* out, uint32_t cap, uint32_t* n);
 int owl_server_info(owl_server server, owl_serve_info_t* info);
 int owl_server_now(owl_server server, double* secs);
+
+/* Autotuning (in place of the reference's tune.cpp / tune.txt, src/tune.cpp:328-738,
+ * src/TuneEntry.cpp:7-84, which choose the FFT length; here the length stays bestFit's and
+ * the tuner chooses how the engine runs it): every W x H split, line block and pass
+ * variant for exponent E at bestFit(E, spec), timed over iters graph-replayed squarings,
+ * fastest first. owl_tune_set / load record choices per FFT length N; owl_gpu_create,
+ * owl_batch_create and the server use them when their flags and lineBlock choose nothing.
+ * Tune file lines: "N width lineBlock flags us". */
+typedef struct {
+  uint32_t width, lineBlock;
+  int flags;          /* OWL_GENERIC | OWL_COL_TWO_REGION */
+  double usPerIter;
+  uint32_t W, H, R;
+  int longCarry;
+  char rowKernel[32], colKernel[32];
+} owl_tune_entry_t;
+int owl_tune_run(uint32_t E, const char* spec, uint32_t iters, owl_tune_entry_t* out, uint32_t cap, uint32_t* n);
+int owl_tune_set(uint32_t N, uint32_t width, uint32_t lineBlock, int flags, double usPerIter);
+/* *found = 0 when no choice is recorded for N */
+int owl_tune_get(uint32_t N, uint32_t* width, uint32_t* lineBlock, int* flags, double* usPerIter, int* found);
+int owl_tune_clear(void);
+int owl_tune_load(const char* path);
+int owl_tune_save(const char* path);
 
 /* Kernel-level access for numerics tests (no reference counterpart; the reference's
  * closest tools are its -verbose self-tests, src/Gpu.cpp:1059-1093).

@@ -11,26 +11,30 @@ from tests.fftmodel import Model
 pytestmark = pytest.mark.gpu
 
 # (spec, bits per word, force the runtime-plan kernels)
-SHAPES = [("64:2:64", 12.0, False), ("256:2:256", 15.0, False), ("256:2:256", 15.0, True), ("256:3:256", 8.0, False),
-          ("1K:3:256", 8.0, False), ("1K:3:256", 8.0, True), ("256:13:256", 8.0, False), ("512:9:256", 8.0, False),
-          ("1K:15:256", 17.6, False), ("1K:15:256", 17.6, True)]
+# (spec, bits per word, kernel variant): "default" = shape-specialized passes where compiled
+# (one-region column pass), "colx" = the two-region column pass, "generic" = runtime-plan kernels
+SHAPES = [("64:2:64", 12.0, "default"), ("256:2:256", 15.0, "default"), ("256:2:256", 15.0, "generic"),
+          ("256:2:256", 15.0, "colx"), ("256:3:256", 8.0, "default"), ("256:3:256", 8.0, "colx"),
+          ("1K:3:256", 8.0, "default"), ("1K:3:256", 8.0, "generic"), ("1K:3:256", 8.0, "colx"),
+          ("256:13:256", 8.0, "default"), ("512:9:256", 8.0, "default"), ("1K:15:256", 17.6, "default"),
+          ("1K:15:256", 17.6, "generic"), ("1K:15:256", 17.6, "colx")]
 
 
 def rel(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-def setup(spec, bpw, generic, seed=1):
+def setup(spec, bpw, variant, seed=1):
     N = owl.best_fit(1000, spec)[1]
     E = workloads.next_prime(int(bpw * N))
-    g = owl.Gpu(E, spec, generic=generic)
+    g = owl.Gpu(E, spec, generic=variant == "generic", col_two_region=variant == "colx")
     m = Model(E, g.W, g.H, g.R)
     return g, m, np.random.default_rng(seed)
 
 
-@pytest.mark.parametrize("spec,bpw,generic", SHAPES)
-def test_lead_in_and_square(spec, bpw, generic):
-    g, m, rng = setup(spec, bpw, generic)
+@pytest.mark.parametrize("spec,bpw,variant", SHAPES)
+def test_lead_in_and_square(spec, bpw, variant):
+    g, m, rng = setup(spec, bpw, variant)
     with g:
         d = m.random_digits(rng)
         g.write_digits(owl.DATA, d.tolist())
@@ -42,9 +46,9 @@ def test_lead_in_and_square(spec, bpw, generic):
         assert rel(sq, m.col_square(f)) < 1e-12
 
 
-@pytest.mark.parametrize("spec,bpw,generic", SHAPES)
-def test_fused_row_pass(spec, bpw, generic):
-    g, m, rng = setup(spec, bpw, generic, seed=2)
+@pytest.mark.parametrize("spec,bpw,variant", SHAPES)
+def test_fused_row_pass(spec, bpw, variant):
+    g, m, rng = setup(spec, bpw, variant, seed=2)
     with g:
         d = m.random_digits(rng)
         g.write_digits(owl.DATA, d.tolist())
@@ -60,9 +64,9 @@ def test_fused_row_pass(spec, bpw, generic):
         assert len(bad) == 0, f"{len(bad)} elements differ; first at HBM index {bad[:8]}"
 
 
-@pytest.mark.parametrize("spec,bpw,generic", SHAPES)
-def test_carry_out_digits(spec, bpw, generic):
-    g, m, rng = setup(spec, bpw, generic, seed=3)
+@pytest.mark.parametrize("spec,bpw,variant", SHAPES)
+def test_carry_out_digits(spec, bpw, variant):
+    g, m, rng = setup(spec, bpw, variant, seed=3)
     with g:
         d = m.random_digits(rng)
         g.write_digits(owl.DATA, d.tolist())
@@ -80,16 +84,20 @@ def test_kernel_selection():
     # the 7.5M-word shape runs the shape-specialized passes; generic=True forces the runtime-plan ones
     E = workloads.next_prime(139_000_000)
     with owl.Gpu(E) as g:
+        assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "coly3840")
+    with owl.Gpu(E, col_two_region=True) as g:
         assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840")
     with owl.Gpu(E, generic=True) as g:
         assert g.row_kernel.startswith("row") and not g.row_kernel.startswith("rowx")
         assert g.col_kernel.startswith("col") and not g.col_kernel.startswith("colx")
 
 
-@pytest.mark.parametrize("spec,bpw", [("1K:3:256", 12.0), ("1K:15:256", 17.6)])
-def test_mul_pass(spec, bpw):
+@pytest.mark.parametrize("spec,bpw,variant", [("1K:3:256", 12.0, "default"), ("1K:3:256", 12.0, "colx"),
+                                              ("1K:15:256", 17.6, "default"), ("1K:15:256", 17.6, "colx"),
+                                              ("1K:15:256", 17.6, "generic")])
+def test_mul_pass(spec, bpw, variant):
     # COL_FWD on a second operand, then COL_MUL, against the model's spectrum product
-    g, m, rng = setup(spec, bpw, False, seed=4)
+    g, m, rng = setup(spec, bpw, variant, seed=4)
     with g:
         a, b = m.random_digits(rng), m.random_digits(rng)
         g.write_digits(owl.AUX, b.tolist())

@@ -60,14 +60,18 @@ def test_ll_res64_vs_gmp(spec, bpw):
         assert out.res64 == res[-1]
 
 
-@pytest.mark.parametrize("spec,bpw,generic", [("64:2:64", 12.0, False), ("256:2:256", 15.0, False),
-                                              ("256:2:256", 15.0, True), ("256:13:256", 8.0, False),
-                                              ("1K:3:256", 12.0, False), ("1K:3:256", 12.0, True)])
-def test_full_residue_vs_gmp(spec, bpw, generic):
+VARIANTS = {"default": {}, "generic": {"generic": True}, "colx": {"col_two_region": True}}
+
+
+@pytest.mark.parametrize("spec,bpw,variant", [("64:2:64", 12.0, "default"), ("256:2:256", 15.0, "default"),
+                                              ("256:2:256", 15.0, "generic"), ("256:2:256", 15.0, "colx"),
+                                              ("256:13:256", 8.0, "default"), ("1K:3:256", 12.0, "default"),
+                                              ("1K:3:256", 12.0, "generic"), ("1K:3:256", 12.0, "colx")])
+def test_full_residue_vs_gmp(spec, bpw, variant):
     E, N = exponent_for(spec, bpw)
-    announce("residue", spec, E, "generic" if generic else "")
+    announce("residue", spec, E, variant)
     want = oracle.gmp_residue(E, 50)
-    with owl.Gpu(E, spec, generic=generic) as g:
+    with owl.Gpu(E, spec, **VARIANTS[variant]) as g:
         g.write_int(owl.DATA, 3)
         g.square_loop(owl.DATA, owl.DATA, 0, 50)
         same = g.read_words(owl.DATA) == want   # (not inside assert: pytest would diff megabit lists)
@@ -159,12 +163,12 @@ def test_config_a_golden(golden):
         assert out.nErrors == 0 and out.maxRoe < 0.4
 
 
-@pytest.mark.parametrize("generic", [False, True])
-def test_config_b_golden_short(golden, generic):
+@pytest.mark.parametrize("variant", ["default", "colx", "generic"])
+def test_config_b_golden_short(golden, variant):
     head, res = golden("B_prp_short")
     E = head["E"]
-    announce("config B prp", E, "generic" if generic else "")
-    with owl.Gpu(E, generic=generic) as g:
+    announce("config B prp", E, variant)
+    with owl.Gpu(E, **VARIANTS[variant]) as g:
         assert g.spec == "1K:15:256:101" and g.N == 7_864_320
         out, got = g.prp(200, block_size=100, check_step=100, res_step=100)
         assert got == [res[100], res[200]]

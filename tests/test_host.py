@@ -95,3 +95,22 @@ def test_server_rejects_bad_requests_before_running():
         assert rid == 1 and s.info()["waiting"] == 1
     with pytest.raises(gpuowl_amd.OwlError):
         gpuowl_amd.Server(mem_budget=1 << 30, block_size=100, check_step=150)
+
+
+def test_tune_table_round_trip(tmp_path):
+    owl = gpuowl_amd
+    owl.tune_clear()
+    assert owl.tune_get(7_864_320) is None
+    owl.tune_set(7_864_320, 1024, 4, 0, 141.5)
+    owl.tune_set(262_144, 256, 2, 4, 22.8)
+    path = tmp_path / "tune.txt"
+    owl.tune_save(path)
+    owl.tune_clear()
+    owl.tune_load(path)
+    assert owl.tune_get(7_864_320) == {"width": 1024, "line_block": 4, "flags": 0, "us": 141.5}
+    assert owl.tune_get(262_144)["flags"] == 4
+    bad = tmp_path / "bad.txt"
+    bad.write_text("262144 256 two 0 1.0\n")
+    with pytest.raises(owl.OwlError):
+        owl.tune_load(bad)
+    owl.tune_clear()
