@@ -66,6 +66,7 @@ SIGNATURES = {
     "owl_hip_copy": [vp, vp, vp, C.c_size_t],
     "owl_hip_fill32": [vp, vp, u32, C.c_size_t],
     "owl_hip_kernel_get": [cp, P(vp)],
+    "owl_hip_kernel_names": [cp, C.c_size_t, P(C.c_size_t)],
     "owl_hip_kernel_max_threads": [vp, P(u32)],
     "owl_hip_launch": [vp, vp, u32, u32, u32, u32, P(vp)],
     "owl_hip_event_create": [P(vp)],

@@ -54,11 +54,11 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   if (W * H != ND) { throw std::invalid_argument("bad FFT geometry"); }
   // a tuned choice for this FFT length applies when the caller chose nothing
   TuneChoice tuned;
-  if (!opt.width && !opt.lineBlock && !opt.generic && !opt.colTwoRegion && TuneTable::get(N, &tuned)) {
+  if (!opt.width && !opt.lineBlock && !opt.generic && !opt.colOneRegion && TuneTable::get(N, &tuned)) {
     opt.width = tuned.width;
     opt.lineBlock = tuned.lineBlock;
     opt.generic = tuned.generic;
-    opt.colTwoRegion = tuned.colTwoRegion;
+    opt.colOneRegion = tuned.colOneRegion;
     tunedLineBlock_ = true;
   }
   if (opt.width) {
@@ -79,15 +79,28 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   if (E % 32 == 0 || E < 64) { throw std::invalid_argument("exponent not supported"); }
   useLongCarry = opt.longCarry;
 
-  // Line block R: as many lines per row-pass group as LDS allows, keeping >= 256 groups,
-  // then longer if needed for the fused carry to be stable: a block's carry-in must die
-  // within its 2R-word chain, (2R - 1) * floor(E/N) >= log2(N)/2 + 4. Below that the
+  // Line block R: the largest one with a shape-specialized row pass that leaves >= 128
+  // groups; otherwise as many lines per group as LDS allows, keeping >= 256 groups, then
+  // longer if needed for the fused carry to be stable. Stable means a block's carry-in dies
+  // within its 2R-word chain: (2R - 1) * floor(E/N) >= log2(N)/2 + 4. Below that the
   // iteration runs with a full carry (lead-out + lead-in) every time, as the reference
-  // does below 12 bits per word (src/Gpu.cpp:522).
+  // does below 12 bits per word (src/Gpu.cpp:522). A tuned choice (Tune.h) overrides this.
   const u32 rMax = std::min<u32>(4096 / W, 16);
   auto fusedStable = [&](u32 r) { return (2.0 * r - 1) * (E / N) >= 0.5 * std::log2(double(N)) + 4; };
+  auto haveKernel = [](const std::string& name) {
+    owl_kernel k = nullptr;
+    return owl_hip_kernel_get(name.c_str(), &k) == 0;
+  };
   R = opt.lineBlock;
   if (R && tunedLineBlock_ && (!fusedStable(R) || H % R || R * W > 4096)) { R = 0; }  // tuned for another exponent
+  if (!R && !opt.generic) {
+    // the largest line block with a shape-specialized row pass that keeps >= 128 groups
+    for (u32 r = rMax; r >= 1 && !R; r /= 2) {
+      if (H % r == 0 && H / r >= 128 && fusedStable(r) && haveKernel("rowx" + std::to_string(W) + "r" + std::to_string(r))) {
+        R = r;
+      }
+    }
+  }
   if (!R) {
     R = rMax;
     while (R > 1 && H / R < 256) { R /= 2; }
@@ -98,17 +111,13 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   useLongCarry = useLongCarry || !fusedStable(R);
 
   // shape-specialized passes when compiled for this shape (and R fits their lane mapping)
-  auto haveKernel = [](const std::string& name) {
-    owl_kernel k = nullptr;
-    return owl_hip_kernel_get(name.c_str(), &k) == 0;
-  };
   rowKernel = "rowx" + std::to_string(W) + "r" + std::to_string(R);
   bool rowFixed = !opt.generic && haveKernel(rowKernel);
-  // column pass: k_coly (one LDS region, pairing in registers) unless the two-region
-  // k_colx is asked for; k_colx's lane mapping needs (H/16) % R == 0
+  // column pass: k_colx (both columns in LDS) unless the one-region k_coly is asked for
+  // or k_colx's lane mapping does not fit ((H/16) % R == 0); then k_coly if compiled
   const std::string colY = "coly" + std::to_string(H), colX = "colx" + std::to_string(H);
-  bool useY = !opt.generic && !opt.colTwoRegion && haveKernel(colY);
-  bool useX = !opt.generic && !useY && haveKernel(colX) && (H / 16) % R == 0;
+  bool useX = !opt.generic && !opt.colOneRegion && haveKernel(colX) && (H / 16) % R == 0;
+  bool useY = !opt.generic && !useX && haveKernel(colY);
   colKernel = useY ? colY : colX;
   bool colFixed = useY || useX;
 
@@ -130,7 +139,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &rowT), "kernel");
-    rowLds = R * (W + W / 16 + 4) * 16;  // rowRegion<W>() in kfixed.hip
+    rowLds = R * (W + W / 16 + 4) * 16 + 64;  // R regions of rowRegion<W>() (kfixed.h) + per-wave round-off
   } else {
     rowKernel = "row" + std::to_string(rowT);
   }
@@ -138,7 +147,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
-    colLds = (useY ? 1 : 2) * (H + H / 16 + 8) * 16;  // colRegion<H>() in kfixed.hip
+    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + 416) * 16;  // colRegion<H>() regions + COLTW_SIZE (kfixed.h)
   } else {
     colKernel = "col" + std::to_string(colT);
   }
@@ -235,6 +244,8 @@ Gpu::~Gpu() {
 void Gpu::finish() { owlCheck(owl_hip_finish(queue_), "finish"); }
 
 u32 Gpu::nextRoeSlot() {
+  if (roeTaken_ >= roeWindow_) { return OWL_ROE_SIZE; }  // past the measured window
+  ++roeTaken_;
   u32 s = std::min<u32>(roePos, OWL_ROE_SIZE - 1);
   ++roePos;
   return s;
@@ -520,6 +531,7 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
   u32 vK = 0;
   double t0 = now();
   readROE();
+  restartRoeWindow();
   double maxRoe = 0;
   while (true) {
     writeIn(DATA, vData);
@@ -572,6 +584,8 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
 
 LLResult Gpu::isPrimeLL(u32 iters, std::vector<u64>* resAt, u32 resStep) {
   LLResult out;
+  readROE();
+  restartRoeWindow();
   writeIn(DATA, makeWords(E, 4));
   double t0 = now();
   bool leadIn = true;
@@ -593,6 +607,8 @@ double Gpu::timePRP(u32 iters, u32 warmup, u32 B) {
   // Like the reference timePRP (src/Gpu.cpp:1337-1398): includes one Gerbicz update per block.
   writeIn(DATA, makeWords(E, 3));
   writeIn(CHECK, makeWords(E, 1));
+  readROE();
+  restartRoeWindow();
   u32 k = 0;
   modMul(CHECK, DATA);
   square(DATA, DATA, true, useLongCarry);
@@ -627,7 +643,7 @@ double Gpu::timeSquares(u32 iters, u32 warmup, bool useGraph) {
   rowLeadIn(DATA);
   for (u32 i = 0; i < warmup; ++i) {
     colSquare();
-    rowFused(false, OWL_ROE_SIZE - 1);
+    rowFused(false, OWL_ROE_SIZE);
   }
   owl_graph graph = nullptr;
   const u32 G = 10;
@@ -635,7 +651,7 @@ double Gpu::timeSquares(u32 iters, u32 warmup, bool useGraph) {
     owlCheck(owl_hip_graph_begin(queue_), "graph begin");
     for (u32 i = 0; i < G; ++i) {
       colSquare();
-      rowFused(false, OWL_ROE_SIZE - 1);
+      rowFused(false, OWL_ROE_SIZE);
     }
     owlCheck(owl_hip_graph_end(queue_, &graph), "graph end");
     owlCheck(owl_hip_graph_launch(graph, queue_), "graph launch");
@@ -652,7 +668,7 @@ double Gpu::timeSquares(u32 iters, u32 warmup, bool useGraph) {
       done += G;
     } else {
       colSquare();
-      rowFused(false, OWL_ROE_SIZE - 1);
+      rowFused(false, OWL_ROE_SIZE);
       ++done;
     }
   }
@@ -664,7 +680,7 @@ double Gpu::timeSquares(u32 iters, u32 warmup, bool useGraph) {
   if (graph) { owl_hip_graph_destroy(graph); }
   // leave the state consistent: one more squaring ending in a lead-out
   colSquare();
-  rowCarryOut(DATA, false, false, OWL_ROE_SIZE - 1);
+  rowCarryOut(DATA, false, false, OWL_ROE_SIZE);
   carryAll(DATA);
   finish();
   return ms * 1e3 / done;

@@ -38,7 +38,7 @@ struct RoeInfo {
 struct GpuOptions {
   bool longCarry = false;  // lead-out + lead-in around every iteration (the reference's -carry long)
   bool generic = false;    // runtime-plan kernels even where a shape-specialized pass exists
-  bool colTwoRegion = false;  // the two-region column pass (k_colx) where k_coly exists
+  bool colOneRegion = false;  // the one-region column pass (k_coly) where k_colx exists
   u32 width = 0;           // W of the W x H split of the FFT; 0 = the reference's (refactored when too tall)
   u32 lineBlock = 0;       // lines per row-pass block; 0 = automatic
   int device = 0;
@@ -97,6 +97,9 @@ public:
   void copy(Buf dst, Buf src);
 
   RoeInfo readROE() { return readROE(roePos); }
+  // Round-off is measured on the next `n` squarings and multiplies (then not, like the
+  // reference's 2,000 iterations per run, src/Gpu.cpp:1459-1464); the run loops restart it.
+  void restartRoeWindow(u32 n = 2000) { roeWindow_ = n, roeTaken_ = 0; }
   RoeInfo readROE(u32 nSlots);   // the first nSlots round-off slots (then cleared)
   u32 roeCount() const { return roePos; }
   void finish();
@@ -182,7 +185,7 @@ private:
   Buffer<JobDesc> jobs_;
   std::vector<std::unique_ptr<Buffer<char>>> owned_;
   std::shared_ptr<const TrigTables> trig_;
-  u32 roePos = 0;
+  u32 roePos = 0, roeWindow_ = 2000, roeTaken_ = 0;
   u32 stairTag_ = 0, lastTag_ = 0, resLo_ = 1, resHi_ = 0;
   u32 faultAt_ = 0;
   bool tunedLineBlock_ = false;

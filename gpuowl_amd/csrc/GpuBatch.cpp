@@ -83,6 +83,11 @@ std::vector<PRPResult> GpuBatch::prp(u32 iters, u32 B, u32 checkStep, std::vecto
   u32 vK = 0;
   u32 totalErrors = 0;
   if (resAt) { resAt->assign(n, {}); }
+  for (Group& g : groups) {
+    u32 cnt = g.lead->roeCount();
+    for (size_t i : g.index) { jobs[i]->readROE(cnt); }
+    g.lead->restartRoeWindow();
+  }
   double t0 = now();
   // round-off slots advance on each group's lead; every job of the group reads that many
   auto readRoe = [&](std::vector<PRPResult>& res) {
@@ -207,7 +212,7 @@ double GpuBatch::timeSquares(u32 iters, u32 warmup, bool useGraph) {
       g.lead->square(Gpu::DATA, Gpu::DATA, true, true, false, false, g.launch());
     } else {
       g.lead->colSquare(g.launch());
-      g.lead->rowFused(false, OWL_ROE_SIZE - 1, g.launch());
+      g.lead->rowFused(false, OWL_ROE_SIZE, g.launch());
     }
   };
   for (u32 i = 0; i < warmup; ++i) {
@@ -242,7 +247,7 @@ double GpuBatch::timeSquares(u32 iters, u32 warmup, bool useGraph) {
   for (Group& g : groups) {
     if (!g.lead->useLongCarry) {
       g.lead->colSquare(g.launch());
-      g.lead->rowCarryOut(Gpu::DATA, false, false, OWL_ROE_SIZE - 1, g.launch());
+      g.lead->rowCarryOut(Gpu::DATA, false, false, OWL_ROE_SIZE, g.launch());
       g.lead->carryAll(Gpu::DATA, g.launch());
     }
   }

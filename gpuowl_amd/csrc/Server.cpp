@@ -103,7 +103,6 @@ void Server::admit() {
       g.writeIn(Gpu::DATA, job->vData);
       g.writeIn(Gpu::CHECK, job->vCheck);
     }
-    g.readROE(OWL_ROE_SIZE);
     bytesInUse_ += job->bytes;
     Group& grp = groupOf(*job);
     if (!grp.lead) { grp.lead = &g; }
@@ -128,7 +127,7 @@ void Server::prepare(Group& g) {
     owlCheck(owl_hip_graph_destroy(g.graph), "graph destroy");
     g.graph = nullptr;
   }
-  if (!g.lead->useLongCarry) { g.graph = g.lead->captureFused(GRAPH_LEN, g.ll, 0, g.launch(), &g.graphLast); }
+  if (!g.lead->useLongCarry) { g.graph = g.lead->captureFused(GRAPH_LEN, g.ll, OWL_ROE_SIZE, g.launch(), &g.graphLast); }
   g.dirty = false;
 }
 
@@ -140,18 +139,19 @@ Gpu::Launch Server::subset(Group& g, const std::vector<Job*>& js) {
   return {g.sub.get(), u32(js.size()), g.lead->queue()};
 }
 
-// count fused squarings (column pass + fused row pass), from captured graphs where possible
-void Server::fused(Group& g, u32 count) {
+// count fused squarings (column pass + fused row pass), from captured graphs where possible;
+// a round that measures round-off (slot 0) launches them directly
+void Server::fused(Group& g, u32 count, u32 roeSlot) {
   Gpu& L = *g.lead;
   Gpu::Launch ln = g.launch();
   u32 i = 0;
-  for (; g.graph && i + GRAPH_LEN <= count; i += GRAPH_LEN) {
+  for (; g.graph && roeSlot >= OWL_ROE_SIZE && i + GRAPH_LEN <= count; i += GRAPH_LEN) {
     owlCheck(owl_hip_graph_launch(g.graph, ln.q), "graph launch");
     L.noteTag(g.graphLast);
   }
   for (; i < count; ++i) {
     L.colSquare(ln);
-    L.rowFused(g.ll, 0, ln);
+    L.rowFused(g.ll, roeSlot, ln);
   }
 }
 
@@ -185,6 +185,10 @@ void Server::runRound(Group& g) {
   std::sort(cuts.begin(), cuts.end());
   cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
 
+  // round-off is measured (slot 0) in the first two rounds of every job, then not
+  bool measure = false;
+  for (Job* j : g.members) { measure = measure || j->k < 2 * B; }
+  const u32 roeSlot = measure ? 0 : OWL_ROE_SIZE;
   u32 pos = 0;
   for (u32 cut : cuts) {
     const u32 n = cut - pos;
@@ -192,9 +196,9 @@ void Server::runRound(Group& g) {
       for (u32 i = 0; i < n; ++i) { L.square(Gpu::DATA, Gpu::DATA, true, true, false, g.ll, ln); }
     } else {
       L.rowLeadInTo(Gpu::DATA, 0, ln);
-      fused(g, n - 1);
+      fused(g, n - 1, roeSlot);
       L.colSquare(ln);
-      L.rowCarryOut(Gpu::DATA, g.ll, false, 0, ln);
+      L.rowCarryOut(Gpu::DATA, g.ll, false, roeSlot, ln);
       L.carryAll(Gpu::DATA, ln);
     }
     for (Job* j : g.members) {

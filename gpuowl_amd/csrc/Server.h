@@ -108,7 +108,7 @@ private:
   void prepare(Group& g);
   Gpu::Launch subset(Group& g, const std::vector<Job*>& js);
   void runRound(Group& g);
-  void fused(Group& g, u32 count);
+  void fused(Group& g, u32 count, u32 roeSlot);
   void verify(Group& g);
   void retire();
   Group& groupOf(Job& j);

@@ -17,7 +17,7 @@ std::map<u32, TuneChoice>& table() {
   static std::map<u32, TuneChoice> t;
   return t;
 }
-int flagsOf(const TuneChoice& c) { return (c.generic ? 2 : 0) | (c.colTwoRegion ? 4 : 0); }
+int flagsOf(const TuneChoice& c) { return (c.generic ? 2 : 0) | (c.colOneRegion ? 4 : 0); }
 }  // namespace
 
 void TuneTable::set(u32 N, const TuneChoice& c) {
@@ -53,7 +53,7 @@ void TuneTable::load(const std::string& path) {
     if (!(is >> N)) { continue; }
     if (!(is >> c.width >> c.lineBlock >> flags >> c.us)) { throw std::invalid_argument("bad tune line: " + line); }
     c.generic = flags & 2;
-    c.colTwoRegion = flags & 4;
+    c.colOneRegion = flags & 4;
     got[N] = c;
   }
   std::lock_guard<std::mutex> lock(mtx);
@@ -87,7 +87,7 @@ std::vector<TuneResult> tuneShape(u32 E, const std::string& spec, u32 iters, u32
         c.width = w;
         c.lineBlock = r;
         c.generic = v == 2;
-        c.colTwoRegion = v == 1;
+        c.colOneRegion = v == 1;
         cands.push_back(c);
       }
     }
@@ -100,7 +100,7 @@ std::vector<TuneResult> tuneShape(u32 E, const std::string& spec, u32 iters, u32
     opt.width = c.width;
     opt.lineBlock = c.lineBlock;
     opt.generic = c.generic;
-    opt.colTwoRegion = c.colTwoRegion;
+    opt.colOneRegion = c.colOneRegion;
     try {
       Gpu g(E, fft, opt);
       auto key = std::make_tuple(g.rowKernel, g.colKernel, g.W, g.R, g.useLongCarry);

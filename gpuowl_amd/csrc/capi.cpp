@@ -69,7 +69,7 @@ int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, 
     GpuOptions opt;
     opt.longCarry = flags & OWL_LONG_CARRY;
     opt.generic = flags & OWL_GENERIC;
-    opt.colTwoRegion = flags & OWL_COL_TWO_REGION;
+    opt.colOneRegion = flags & OWL_COL_ONE_REGION;
     opt.lineBlock = lineBlock;
     *out = new owl_gpu_s{new Gpu(E, fft, opt)};
   });
@@ -227,7 +227,7 @@ int owl_gpu_time_kernels(owl_gpu g, uint32_t iters, double* usCol, double* usRow
     for (uint32_t i = 0; i < iters; ++i) {
       G.colSquare();
       owlCheck(owl_hip_event_record(ev[2 * i + 1], q), "event");
-      G.rowFused(false, OWL_ROE_SIZE - 1);
+      G.rowFused(false, OWL_ROE_SIZE);
       owlCheck(owl_hip_event_record(ev[2 * i + 2], q), "event");
     }
     double col = 0, row = 0;
@@ -242,7 +242,7 @@ int owl_gpu_time_kernels(owl_gpu g, uint32_t iters, double* usCol, double* usRow
     *usCol = col * 1e3 / iters;
     *usRow = row * 1e3 / iters;
     G.colSquare();
-    G.rowCarryOut(Gpu::DATA, false, false, OWL_ROE_SIZE - 1);
+    G.rowCarryOut(Gpu::DATA, false, false, OWL_ROE_SIZE);
     G.carryAll(Gpu::DATA);
     G.finish();
   });
@@ -279,7 +279,7 @@ int owl_tune_run(uint32_t E, const char* spec, uint32_t iters, owl_tune_entry_t*
       owl_tune_entry_t& o = out[i];
       o.width = r.choice.width;
       o.lineBlock = r.choice.lineBlock;
-      o.flags = (r.choice.generic ? OWL_GENERIC : 0) | (r.choice.colTwoRegion ? OWL_COL_TWO_REGION : 0);
+      o.flags = (r.choice.generic ? OWL_GENERIC : 0) | (r.choice.colOneRegion ? OWL_COL_ONE_REGION : 0);
       o.usPerIter = r.choice.us;
       o.W = r.W, o.H = r.H, o.R = r.R;
       o.longCarry = r.longCarry;
@@ -296,7 +296,7 @@ int owl_tune_set(uint32_t N, uint32_t width, uint32_t lineBlock, int flags, doub
     c.width = width;
     c.lineBlock = lineBlock;
     c.generic = flags & OWL_GENERIC;
-    c.colTwoRegion = flags & OWL_COL_TWO_REGION;
+    c.colOneRegion = flags & OWL_COL_ONE_REGION;
     c.us = us;
     TuneTable::set(N, c);
   });
@@ -308,7 +308,7 @@ int owl_tune_get(uint32_t N, uint32_t* width, uint32_t* lineBlock, int* flags, d
     *found = TuneTable::get(N, &c);
     *width = c.width;
     *lineBlock = c.lineBlock;
-    *flags = (c.generic ? OWL_GENERIC : 0) | (c.colTwoRegion ? OWL_COL_TWO_REGION : 0);
+    *flags = (c.generic ? OWL_GENERIC : 0) | (c.colOneRegion ? OWL_COL_ONE_REGION : 0);
     *us = c.us;
   });
 }
@@ -337,7 +337,7 @@ int owl_server_create(uint64_t memBudget, uint32_t maxJobs, uint32_t blockSize, 
     o.spec = spec ? spec : "";
     o.gpu.longCarry = flags & OWL_LONG_CARRY;
     o.gpu.generic = flags & OWL_GENERIC;
-    o.gpu.colTwoRegion = flags & OWL_COL_TWO_REGION;
+    o.gpu.colOneRegion = flags & OWL_COL_ONE_REGION;
     *out = new owl_server_s{new Server(o), {}};
   });
 }
@@ -413,7 +413,7 @@ int owl_batch_create(const uint32_t* exponents, uint32_t n, const char* spec, in
     GpuOptions opt;
     opt.longCarry = flags & OWL_LONG_CARRY;
     opt.generic = flags & OWL_GENERIC;
-    opt.colTwoRegion = flags & OWL_COL_TWO_REGION;
+    opt.colOneRegion = flags & OWL_COL_ONE_REGION;
     *out = new owl_batch_s{new GpuBatch(std::vector<u32>(exponents, exponents + n), spec ? spec : "", opt)};
   });
 }
@@ -512,9 +512,9 @@ int owl_gpu_pass(owl_gpu g, int pass, int arg) {
     switch (pass) {
       case OWL_PASS_LEADIN: G.rowLeadIn(buf(arg)); break;
       case OWL_PASS_SQUARE: G.colSquare(); break;
-      case OWL_PASS_FUSED: G.rowFused(arg != 0, OWL_ROE_SIZE - 1); break;
+      case OWL_PASS_FUSED: G.rowFused(arg != 0, OWL_ROE_SIZE); break;
       case OWL_PASS_CARRYOUT:
-        G.rowCarryOut(buf(arg), false, false, OWL_ROE_SIZE - 1);
+        G.rowCarryOut(buf(arg), false, false, OWL_ROE_SIZE);
         G.carryAll(buf(arg));
         break;
       case OWL_PASS_SPECTRUM: G.colFwdSpec(); break;

@@ -18,7 +18,13 @@ struct OwlKernelEntry {
   unsigned maxThreads;
 };
 extern "C" const OwlKernelEntry owl_kernel_table[];
-extern "C" const OwlKernelEntry owl_kernel_table_fixed[];
+extern "C" const OwlKernelEntry owl_kernel_table_row_a[];
+extern "C" const OwlKernelEntry owl_kernel_table_row_b[];
+extern "C" const OwlKernelEntry owl_kernel_table_row_c[];
+extern "C" const OwlKernelEntry owl_kernel_table_colx_a[];
+extern "C" const OwlKernelEntry owl_kernel_table_colx_b[];
+extern "C" const OwlKernelEntry owl_kernel_table_coly_a[];
+extern "C" const OwlKernelEntry owl_kernel_table_coly_b[];
 
 struct owl_queue_s {
   hipStream_t stream;
@@ -143,6 +149,26 @@ int owl_hip_copy(owl_queue queue, void* dst, const void* src, size_t bytes) {
   return 0;
 }
 
+namespace {
+const std::initializer_list<const OwlKernelEntry*> kTables = {
+    owl_kernel_table,       owl_kernel_table_row_a,  owl_kernel_table_row_b,  owl_kernel_table_row_c,
+    owl_kernel_table_colx_a, owl_kernel_table_colx_b, owl_kernel_table_coly_a, owl_kernel_table_coly_b};
+}  // namespace
+
+int owl_hip_kernel_names(char* out, size_t cap, size_t* needed) {
+  std::string all;
+  for (const OwlKernelEntry* table : kTables) {
+    for (const OwlKernelEntry* e = table; e->name; ++e) {
+      if (!all.empty()) { all += ','; }
+      all += e->name;
+    }
+  }
+  if (needed) { *needed = all.size() + 1; }
+  if (!out || cap < all.size() + 1) { return fail(-1, "kernel name buffer too small"); }
+  memcpy(out, all.c_str(), all.size() + 1);
+  return 0;
+}
+
 int owl_hip_kernel_get(const char* name, owl_kernel* kernel) {
   static std::mutex mtx;
   static std::unordered_map<std::string, owl_kernel_s*> cache;
@@ -152,7 +178,7 @@ int owl_hip_kernel_get(const char* name, owl_kernel* kernel) {
     *kernel = it->second;
     return 0;
   }
-  for (const OwlKernelEntry* table : {owl_kernel_table, owl_kernel_table_fixed}) {
+  for (const OwlKernelEntry* table : kTables) {
     for (const OwlKernelEntry* e = table; e->name; ++e) {
       if (!strcmp(e->name, name)) {
         auto* k = new owl_kernel_s{*e};

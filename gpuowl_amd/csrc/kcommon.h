@@ -1,5 +1,5 @@
 // Device helpers shared by the generic (runtime-plan) kernels in kernels.hip and
-// the shape-specialized kernels in kfixed.hip: global-address-space access,
+// the shape-specialized kernels in kfixed.h: global-address-space access,
 // complex arithmetic, in-register DFTs, IBDWT word weights, rounding, round-off
 // reporting, tagged stairway carries, mid twiddles and the real-signal pairing.
 #pragma once
@@ -225,10 +225,14 @@ __device__ __forceinline__ int64_t roundWord(double v, float& roe) {
   return __double_as_longlong(t) - OWL_RNDVAL_BITS;
 }
 
+// A launch measures round-off only when its slot is < OWL_ROE_SIZE (the host measures a
+// window of iterations, as the reference does, src/Gpu.cpp:1459-1464); the slot is
+// uniform, so unmeasured launches skip the reduction and the atomic.
 __device__ __forceinline__ void roeReport(const JobDesc& J, uint32_t slot, float roe) {
+  if (slot >= OWL_ROE_SIZE) { return; }
   // wave-level max, then one atomic per wave (float bits of non-negative values order as uints)
   for (int off = 32; off > 0; off >>= 1) { roe = fmaxf(roe, __shfl_xor(roe, off, 64)); }
-  if ((threadIdx.x & 63) == 0 && slot < OWL_ROE_SIZE) { atomicMax(J.roe + slot, __float_as_uint(roe)); }
+  if ((threadIdx.x & 63) == 0) { atomicMax(J.roe + slot, __float_as_uint(roe)); }
 }
 
 #define OWL_SPIN_LIMIT (1u << 24)
@@ -253,6 +257,45 @@ __device__ __forceinline__ uint64_t pollCarry(const JobDesc& J, uint64_t* p, uin
     v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return v;
+}
+
+// N polls issued together: every carry's first load is in flight at once, then only the
+// ones whose tag is not current yet are loaded again.
+template<int N>
+__device__ __forceinline__ void pollCarries(const JobDesc& J, uint64_t* const* p, uint32_t tag, uint64_t* v) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) { v[k] = __hip_atomic_load(p[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+  uint32_t n = 0;
+  while (true) {
+    bool ready = true;
+#pragma unroll
+    for (int k = 0; k < N; ++k) { ready = ready && (uint32_t)(v[k] >> 48) == tag; }
+    if (ready) { break; }
+    __builtin_amdgcn_s_sleep(1);
+    if (++n >= OWL_SPIN_LIMIT) {
+      atomicOr(J.flags + 2, 1u);  // hand-off timed out: flagged to the host, kernel still exits
+      break;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      if ((uint32_t)(v[k] >> 48) != tag) { v[k] = __hip_atomic_load(p[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    }
+  }
+}
+
+// Round-off of a workgroup: each wave's max goes to part[wave]; after a barrier, thread 0
+// folds them and makes the group's single atomic (one per group instead of one per wave).
+__device__ __forceinline__ void roeWave(uint32_t slot, float roe, float* part) {
+  if (slot >= OWL_ROE_SIZE) { return; }
+  for (int off = 32; off > 0; off >>= 1) { roe = fmaxf(roe, __shfl_xor(roe, off, 64)); }
+  if ((threadIdx.x & 63) == 0) { part[threadIdx.x >> 6] = roe; }
+}
+__device__ __forceinline__ void roeGroup(const JobDesc& J, uint32_t slot, const float* part, int nWaves) {
+  if (threadIdx.x == 0 && slot < OWL_ROE_SIZE) {
+    float m = part[0];
+    for (int w = 1; w < nWaves; ++w) { m = fmaxf(m, part[w]); }
+    atomicMax(J.roe + slot, __float_as_uint(m));
+  }
 }
 
 // ---------------------------------------------------------------- column pass

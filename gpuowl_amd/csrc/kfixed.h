@@ -1,4 +1,5 @@
-// Shape-specialized row and column passes for gfx950.
+// Shape-specialized row and column passes for gfx950 (templates; the kfix_*.hip files
+// instantiate them for the compiled shapes and export their name tables).
 //
 // Same algorithm, layout and modes as k_row / k_col in kernels.hip, with the
 // FFT plan fixed at compile time (no integer division, no radix dispatch, every
@@ -16,6 +17,8 @@
 // line (row pass) or column (column pass) regions start 4 or 8 elements apart
 // in the 64-bank cycle, so that the slot-order load and store phases, whose 16
 // lanes of a b128 access span several regions, spread over the banks.
+#pragma once
+
 #include "kcommon.h"
 
 __device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
@@ -30,9 +33,11 @@ template<int H> constexpr int colRegion() { return H + H / 16 + 8; }
 // one wave complete in order, so such a stage needs no workgroup barrier, only a compiler
 // fence. The caller puts a __syncthreads() after the last stage when the next phase
 // reads across waves.
-template<int L, int REG, int NFFT, int T, bool INV, int R, int Ns, bool PRIV = false>
-__device__ __forceinline__ void xstage(T2* lds, const T2* __restrict__ tw, int tid) {
-  constexpr bool priv = PRIV && (T == 64 || L / R == 64);
+template<int L, int T, int R, bool PRIV> constexpr bool stagePrivate() { return PRIV && (T == 64 || L / R == 64); }
+
+template<int L, int REG, int NFFT, int T, bool INV, int R, int Ns, bool PRIV = false, class TW>
+__device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
+  constexpr bool priv = stagePrivate<L, T, R, PRIV>();
   constexpr int NB = L / R;
   constexpr int TOT = NB * NFFT;
   constexpr int MB = (TOT + T - 1) / T;
@@ -50,7 +55,7 @@ __device__ __forceinline__ void xstage(T2* lds, const T2* __restrict__ tw, int t
         const int p = j % Ns;
 #pragma unroll
         for (int k = 1; k < R; ++k) {
-          T2 w = gld(tw + p * STRIDE * k);
+          T2 w = tw(p * STRIDE * k);
           v[m][k] = INV ? cmulc(v[m][k], w) : cmul(v[m][k], w);
         }
       }
@@ -83,24 +88,74 @@ __device__ __forceinline__ void xstage(T2* lds, const T2* __restrict__ tw, int t
 
 template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, int Ns, int... Rs> struct XStages;
 template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, int Ns> struct XStages<L, REG, NFFT, T, INV, PRIV, Ns> {
-  __device__ __forceinline__ static void run(T2*, const T2*, int) {}
+  template<class TW> __device__ __forceinline__ static void run(T2*, const TW&, int) {}
 };
 template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, int Ns, int R, int... Rs>
 struct XStages<L, REG, NFFT, T, INV, PRIV, Ns, R, Rs...> {
-  __device__ __forceinline__ static void run(T2* lds, const T2* tw, int tid) {
+  template<class TW> __device__ __forceinline__ static void run(T2* lds, const TW& tw, int tid) {
     xstage<L, REG, NFFT, T, INV, R, Ns, PRIV>(lds, tw, tid);
+    if constexpr (sizeof...(Rs) > 0) {
+      // a workgroup-wide stage after a wave-private one reads what other waves wrote
+      constexpr int next[] = {Rs...};
+      if constexpr (stagePrivate<L, T, R, PRIV>() && !stagePrivate<L, T, next[0], PRIV>()) { __syncthreads(); }
+    }
     XStages<L, REG, NFFT, T, INV, PRIV, Ns * R, Rs...>::run(lds, tw, tid);
   }
 };
 
+// exp(-2 pi i m / L) from a table in global memory
+struct GlobalTw {
+  const T2* t;
+  __device__ __forceinline__ T2 operator()(int m) const { return gld(t + m); }
+};
+
+// Column-pass twiddles staged in LDS after the data regions (COLTW_SIZE entries):
+// w_H^m = TA[m & 63] * TB[m >> 6] for m < H, and for column h of the pair the mid
+// twiddles w_ND^(a*l) = MA[h][l & 63] * MB[h][l >> 6] and S[h][k] = w_ND^(a*k*H/16).
+// Each twiddle is a product of two correctly rounded table entries.
+constexpr int COLTW_SIZE = 64 + 64 + 2 * (64 + 64 + 16);
+struct ColTw {
+  const T2* tab;
+  __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[m & 63], tab[64 + (m >> 6)]); }
+  __device__ __forceinline__ T2 mid(int h, int l) const {
+    const T2* p = tab + 128 + h * 144;
+    return cmul(p[l & 63], p[64 + (l >> 6)]);
+  }
+  __device__ __forceinline__ T2 s(int h, int k) const { return tab[128 + h * 144 + 128 + k]; }
+};
+template<int H>
+__device__ __forceinline__ void fillColTw(const JobDesc& J, T2* tab, int a0, int a1, int tid, int T) {
+  static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
+  for (int i = tid; i < COLTW_SIZE; i += T) {
+    T2 v = make_double2(0.0, 0.0);
+    if (i < 64) {
+      v = gld(J.twCol + i);
+    } else if (i < 128) {
+      if (i - 64 < H / 64) { v = gld(J.twCol + 64 * (i - 64)); }
+    } else {
+      const int h = (i - 128) / 144, r = (i - 128) % 144;
+      const int a = h ? a1 : a0;
+      if (r < 64) {
+        v = gld(J.twMidA + a * 64 + r);
+      } else if (r < 128) {
+        if (r - 64 < H / 64) { v = gld(J.twMidB + a * J.nHi + (r - 64)); }
+      } else {
+        v = gld(J.twMidS + a * 16 + (r - 128));
+      }
+    }
+    tab[i] = v;
+  }
+}
+
 // ---------------------------------------------------------------- column pass
-// Grid (W/2, batch), T threads, LDS 2*colRegion<H>()*16 bytes. Modes as k_col.
-// J.twMidS[a*C0 + k] = exp(-2 pi i a*k*(H/C0) / ND).
+// Grid (W/2, batch), T threads, LDS (2*colRegion<H>() + COLTW_SIZE)*16 bytes. Modes as k_col.
+// Twiddles come from the LDS table (ColTw).
 template<int H, int T, int C0, int C1, int C2>
 __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, int mode, int fi, int oi) {
-  static_assert(C0 * C1 * C2 == H, "plan");
+  static_assert(C0 * C1 * C2 == H && C0 == 16, "plan");
   extern __shared__ T2 lds[];
   constexpr int REG = colRegion<H>();
+  const ColTw tw{lds + 2 * REG};
   constexpr int NB0 = H / C0;
   constexpr int NBT = 2 * NB0;
   constexpr int MB0 = (NBT + T - 1) / T;
@@ -111,6 +166,8 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
   T2* io = J.freq[fi];
   const size_t colBase = (size_t)2 * q * R;
   const size_t blockStride = (size_t)W * R;
+  fillColTw<H>(J, lds + 2 * REG, a0, a1, tid, T);
+  __syncthreads();
 
   // forward stage 0 (radix C0, Ns = 1), fused with the load and the mid twiddle.
   // Lane order within 2R lanes: (column h, line-in-block r), so 2R lanes read 2R*16
@@ -121,15 +178,13 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
     if (NBT % T == 0 || g < NBT) {
       const int h = (g >> lgR) & 1;
       const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
-      const int a = h ? a1 : a0;
-      const T2 base = midTw(J, a, j);
-      const T2* S = J.twMidS + a * C0;
+      const T2 base = tw.mid(h, j);
       T2 v[C0];
 #pragma unroll
       for (int k = 0; k < C0; ++k) {
         const int l = j + k * NB0;
         T2 x = gld(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)));
-        v[k] = cmul(x, k ? cmul(base, gld(S + k)) : base);
+        v[k] = cmul(x, k ? cmul(base, tw.s(h, k)) : base);
       }
       Dft<C0, false>::run(v);
 #pragma unroll
@@ -137,7 +192,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
     }
   }
   __syncthreads();
-  XStages<H, REG, 2, T, false, false, C0, C1, C2>::run(lds, J.twCol, tid);
+  XStages<H, REG, 2, T, false, false, C0, C1, C2>::run(lds, tw, tid);
 
   const int per = 2 * R;
   if (mode == COL_FWD) {
@@ -153,13 +208,12 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
 
   const T2* other = (mode == COL_MUL) ? J.freq[oi] : nullptr;
   auto spos = [&](int h, int b) -> size_t { return (size_t)(b >> lgR) * blockStride + colBase + (h << lgR) + (b & (R - 1)); };
-  const T2* twCol = J.twCol;
   if (q) {
     const T2 wa = gld(J.twPair + a0);
     for (int b = tid; b < H; b += T) {
       int bp = H - 1 - b;
       T2 X = lds[padi(b)], Xp = lds[REG + padi(bp)];
-      T2 w = cmul(wa, gld(twCol + b));
+      T2 w = cmul(wa, tw(b));
       if (other) {
         pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(1, bp)), w);
       } else {
@@ -174,7 +228,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
       if (i <= H / 2) {
         int b = i, bp = (H - i) % H;
         T2 X = lds[padi(b)], Xp = lds[padi(bp)];
-        T2 w = gld(twCol + b);
+        T2 w = tw(b);
         if (other) {
           pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(0, bp)), w);
         } else {
@@ -185,7 +239,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
       } else {
         int b = i - H / 2 - 1, bp = H - 1 - b;
         T2 X = lds[REG + padi(b)], Xp = lds[REG + padi(bp)];
-        T2 w = cmul(wa, gld(twCol + b));
+        T2 w = cmul(wa, tw(b));
         if (other) {
           pairMul(X, Xp, gld(other + spos(1, b)), gld(other + spos(1, bp)), w);
         } else {
@@ -197,7 +251,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
     }
   }
   __syncthreads();
-  XStages<H, REG, 2, T, true, false, 1, C2, C1>::run(lds, twCol, tid);
+  XStages<H, REG, 2, T, true, false, 1, C2, C1>::run(lds, tw, tid);
 
   // last inverse stage (radix C0, Ns = NB0), fused with the conjugate mid twiddle and the store
 #pragma unroll
@@ -206,20 +260,18 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
     if (NBT % T == 0 || g < NBT) {
       const int h = (g >> lgR) & 1;
       const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
-      const int a = h ? a1 : a0;
       T2 v[C0];
 #pragma unroll
       for (int k = 0; k < C0; ++k) { v[k] = lds[h * REG + padi(j + k * NB0)]; }
 #pragma unroll
-      for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], gld(twCol + j * k)); }
+      for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw(j * k)); }
       Dft<C0, true>::run(v);
-      const T2 base = midTw(J, a, j);
-      const T2* S = J.twMidS + a * C0;
+      const T2 base = tw.mid(h, j);
 #pragma unroll
       for (int k = 0; k < C0; ++k) {
         const int l = j + k * NB0;
         gst(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)),
-            cmulc(v[k], k ? cmul(base, gld(S + k)) : base));
+            cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base));
       }
     }
   }
@@ -233,7 +285,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
 // X1[H-1-b], is X1[(T-1-t) + T*(C2-1-k)] in the same thread. The inverse's first stage
 // (radix C2, Ns = 1) reads exactly those registers. The pair q = 0 (columns 0 and W/2,
 // each paired with itself) goes through LDS. Plan: forward [16, C1, C2], inverse
-// [C2, C1, 16]; J.twMidS[a*16 + k] = exp(-2 pi i a*k*(H/16) / ND).
+// [C2, C1, 16]; LDS (colRegion<H>() + COLTW_SIZE)*16 bytes, twiddles from the LDS table.
 template<int H, int C1, int C2>
 __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ jobs, int mode, int fi, int oi) {
   constexpr int C0 = 16;
@@ -242,6 +294,7 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
   static_assert(C0 * C1 * C2 == H && T % 64 == 0 && NB0 <= T, "plan");
   constexpr int REG = colRegion<H>();
   extern __shared__ T2 lds[];
+  const ColTw tw{lds + REG};
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, lgR = J.lgR;
   const int q = blockIdx.x, tid = threadIdx.x;
@@ -249,67 +302,66 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
   T2* io = J.freq[fi];
   const size_t colBase = (size_t)2 * q * R;
   const size_t blockStride = (size_t)W * R;
-  const T2* twCol = J.twCol;
   auto gpos = [&](int h, int l) -> size_t {
     return (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1));
   };
   const int jl1 = q ? T - 1 - tid : tid;  // column 1's last-stage butterfly
+  fillColTw<H>(J, lds + REG, a0, a1, tid, T);
+  __syncthreads();
 
   T2 x0[C2], x1[C2];
 
   // forward FFT of column h (mid twiddle for column index a), last stage for butterfly jl
-  auto forward = [&](int h, int a, int jl, T2* out) {
+  auto forward = [&](int h, int jl, T2* out) {
     if (tid < NB0) {
       const int j = tid;
-      const T2 base = midTw(J, a, j);
-      const T2* S = J.twMidS + a * C0;
+      const T2 base = tw.mid(h, j);
       T2 v[C0];
 #pragma unroll
       for (int k = 0; k < C0; ++k) {
         T2 x = gld(io + gpos(h, j + k * NB0));
-        v[k] = cmul(x, k ? cmul(base, gld(S + k)) : base);
+        v[k] = cmul(x, k ? cmul(base, tw.s(h, k)) : base);
       }
       Dft<C0, false>::run(v);
 #pragma unroll
       for (int k = 0; k < C0; ++k) { lds[padi(j * C0 + k)] = v[k]; }
     }
     __syncthreads();
-    XStages<H, REG, 1, T, false, false, C0, C1>::run(lds, twCol, tid);
+    XStages<H, REG, 1, T, false, false, C0, C1>::run(lds, tw, tid);
 #pragma unroll
     for (int k = 0; k < C2; ++k) { out[k] = lds[padi(jl + k * T)]; }
 #pragma unroll
-    for (int k = 1; k < C2; ++k) { out[k] = cmul(out[k], gld(twCol + jl * k)); }
+    for (int k = 1; k < C2; ++k) { out[k] = cmul(out[k], tw(jl * k)); }
     Dft<C2, false>::run(out);
     __syncthreads();
   };
 
   // inverse FFT of column h from registers (butterfly jl of the first stage), then store
-  auto inverse = [&](int h, int a, int jl, T2* in) {
+  auto inverse = [&](int h, int jl, T2* in) {
     Dft<C2, true>::run(in);
 #pragma unroll
     for (int k = 0; k < C2; ++k) { lds[padi(jl * C2 + k)] = in[k]; }
     __syncthreads();
-    XStages<H, REG, 1, T, true, false, C2, C1>::run(lds, twCol, tid);
+    XStages<H, REG, 1, T, true, false, C2, C1>::run(lds, tw, tid);
     if (tid < NB0) {
       const int j = tid;
       T2 v[C0];
 #pragma unroll
       for (int k = 0; k < C0; ++k) { v[k] = lds[padi(j + k * NB0)]; }
 #pragma unroll
-      for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], gld(twCol + j * k)); }
+      for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw(j * k)); }
       Dft<C0, true>::run(v);
-      const T2 base = midTw(J, a, j);
-      const T2* S = J.twMidS + a * C0;
+      const T2 base = tw.mid(h, j);
 #pragma unroll
       for (int k = 0; k < C0; ++k) {
-        gst(io + gpos(h, j + k * NB0), cmulc(v[k], k ? cmul(base, gld(S + k)) : base));
+        gst(io + gpos(h, j + k * NB0), cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base));
       }
     }
     __syncthreads();
   };
 
-  forward(0, a0, tid, x0);
-  forward(1, a1, jl1, x1);
+  forward(0, tid, x0);
+  forward(1, jl1, x1);
 
   if (mode == COL_FWD) {
 #pragma unroll
@@ -326,7 +378,7 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
 #pragma unroll
     for (int k = 0; k < C2; ++k) {
       const int b = tid + k * T, bp = H - 1 - b;
-      const T2 w = cmul(wa, gld(twCol + b));
+      const T2 w = cmul(wa, tw(b));
       if (other) {
         pairMul(x0[k], x1[C2 - 1 - k], gld(other + gpos(0, b)), gld(other + gpos(1, bp)), w);
       } else {
@@ -346,7 +398,7 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
       for (int b = tid; b <= last; b += T) {
         const int bp = h ? H - 1 - b : (H - b) % H;
         T2 X = lds[padi(b)], Xp = lds[padi(bp)];
-        const T2 w = h ? cmul(wa, gld(twCol + b)) : gld(twCol + b);
+        const T2 w = h ? cmul(wa, tw(b)) : tw(b);
         if (other) {
           pairMul(X, Xp, gld(other + gpos(h, b)), gld(other + gpos(h, bp)), w);
         } else {
@@ -362,8 +414,8 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
     }
   }
 
-  inverse(0, a0, tid, x0);
-  inverse(1, a1, jl1, x1);
+  inverse(0, tid, x0);
+  inverse(1, jl1, x1);
 }
 
 // ---------------------------------------------------------------- row pass
@@ -377,6 +429,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   constexpr int LW = R * W;
   constexpr int REG = rowRegion<W>();
   extern __shared__ T2 lds[];
+  float* roePart = reinterpret_cast<float*>(lds + R * REG);  // one float per wave, after the line regions
   const JobDesc& J = jobs[blockIdx.y];
   const int nb = J.nb;
   const int g = blockIdx.x;
@@ -407,7 +460,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       lds[r * REG + padi(slotFreq(s, W))] = gld(blk + e);
     }
     __syncthreads();
-    XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, J.twRow, tid);
+    XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, GlobalTw{J.twRow}, tid);
     __syncthreads();
     // last inverse stage (radix F0, Ns = T): butterfly tid of each line -> columns tid + k*T
 #pragma unroll
@@ -446,9 +499,8 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
         d[r][k] = make_int2(dd[0], dd[1]);
       }
     }
-    roeReport(J, roeSlot, roe);
-
     if (mode == ROW_CARRYOUT) {
+      roeReport(J, roeSlot, roe);
       int2* dst = J.words[wi] + (size_t)lb * LW;
 #pragma unroll
       for (int k = 0; k < F0; ++k) {
@@ -467,14 +519,29 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
                            __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if (g == 0) { return; }
+    if (g == 0) {
+      roeReport(J, roeSlot, roe);
+      return;
+    }
+    roeWave(roeSlot, roe, roePart);
     const int pred = g - 1;
+    uint64_t* src[F0];
+    uint64_t got[F0];
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
       const int c = tid + k * T;
-      const int srcCol = (g == nb) ? (c == 0 ? W - 1 : c - 1) : c;
-      int64_t carry = unpackCarry(pollCarry(J, J.stair + (size_t)pred * W + srcCol, tag));
-      if (g == nb) { pollCarry(J, J.stair + c, tag); }  // group 0 has read block 0
+      src[k] = J.stair + (size_t)pred * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
+    }
+    pollCarries<F0>(J, src, tag, got);
+    if (g == nb) {  // group 0 has read block 0
+#pragma unroll
+      for (int k = 0; k < F0; ++k) { src[k] = J.stair + tid + k * T; }
+      uint64_t seen[F0];
+      pollCarries<F0>(J, src, tag, seen);
+    }
+#pragma unroll
+    for (int k = 0; k < F0; ++k) {
+      int64_t carry = unpackCarry(got[k]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (carry) {
@@ -512,13 +579,14 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
     Dft<F0, false>::run(v[r]);
   }
   __syncthreads();  // every thread has finished reading the inverse's LDS
+  if (mode == ROW_FUSED) { roeGroup(J, roeSlot, roePart, T / 64); }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
 #pragma unroll
     for (int k = 0; k < F0; ++k) { lds[r * REG + padi(tid * F0 + k)] = v[r][k]; }
   }
   __syncthreads();
-  XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, J.twRow, tid);
+  XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, GlobalTw{J.twRow}, tid);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < LW / T; ++i) {
@@ -528,24 +596,9 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   }
 }
 
-// ---------------------------------------------------------------- table
+// ---------------------------------------------------------------- name tables
 struct OwlKernelEntry {
   const char* name;
   const void* fn;
   unsigned maxThreads;
-};
-
-// Names: colx<H> and rowx<W>r<R>; the host uses them when the shape matches
-// (column plan [C0,C1,C2] with C0 = 16 for twMidS, row plan [F0,F1,F2]).
-extern "C" const OwlKernelEntry owl_kernel_table_fixed[] = {
-    {"colx3840", (const void*)&k_colx<3840, 512, 16, 16, 15>, 512},
-    {"colx768", (const void*)&k_colx<768, 128, 16, 16, 3>, 128},
-    {"colx512", (const void*)&k_colx<512, 64, 16, 16, 2>, 64},
-    {"coly3840", (const void*)&k_coly<3840, 16, 15>, 256},
-    {"coly768", (const void*)&k_coly<768, 4, 12>, 64},
-    {"coly512", (const void*)&k_coly<512, 4, 8>, 64},
-    {"rowx1024r4", (const void*)&k_rowx<1024, 4, 4, 16, 16>, 256},
-    {"rowx1024r2", (const void*)&k_rowx<1024, 2, 4, 16, 16>, 256},
-    {"rowx256r2", (const void*)&k_rowx<256, 2, 4, 4, 16>, 64},
-    {nullptr, nullptr, 0},
 };

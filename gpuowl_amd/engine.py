@@ -11,6 +11,16 @@ def n_words(E):
     return (E - 1) // 32 + 1
 
 
+def kernel_names():
+    """Names of every compiled kernel (the runtime-plan ones and the shape-specialized ones)."""
+    lib = load()
+    need = C.c_size_t()
+    lib.owl_hip_kernel_names(None, 0, C.byref(need))
+    buf = C.create_string_buffer(need.value)
+    check(lib.owl_hip_kernel_names(buf, need.value, C.byref(need)))
+    return buf.value.decode().split(",")
+
+
 def best_fit(E, spec=""):
     """FFTConfig::bestFit: (spec, N, maxExp) of the FFT chosen for exponent E."""
     lib = load()
@@ -30,12 +40,12 @@ def int_to_words(x, E):
 
 class Gpu:
     """One exponent on the GPU: Gpu(E, spec="", long_carry=False, line_block=0, generic=False,
-    col_two_region=False)."""
+    col_one_region=False)."""
 
-    def __init__(self, E, spec="", long_carry=False, line_block=0, generic=False, col_two_region=False):
+    def __init__(self, E, spec="", long_carry=False, line_block=0, generic=False, col_one_region=False):
         self.lib = load()
         self.h = C.c_void_p()
-        flags = (1 if long_carry else 0) | (2 if generic else 0) | (4 if col_two_region else 0)
+        flags = (1 if long_carry else 0) | (2 if generic else 0) | (4 if col_one_region else 0)
         check(self.lib.owl_gpu_create(E, (spec or "").encode(), flags, line_block, C.byref(self.h)))
         info = GpuInfo()
         check(self.lib.owl_gpu_info(self.h, C.byref(info)))

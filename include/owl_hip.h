@@ -52,6 +52,9 @@ int owl_hip_fill32(owl_queue queue, void* dst, uint32_t value, size_t count);
 /* Kernels: loadKernel by name / run (src/clwrap.h:106-112, src/clwrap.cpp:314-323).
  * args is an array of pointers to the argument values, in kernel parameter order. */
 int owl_hip_kernel_get(const char* name, owl_kernel* kernel);
+/* Comma-separated names of every compiled kernel (no clwrap counterpart: the reference
+   compiles kernels at run time); *needed = bytes including the terminator. */
+int owl_hip_kernel_names(char* out, size_t cap, size_t* needed);
 int owl_hip_kernel_max_threads(owl_kernel kernel, uint32_t* maxThreads);
 int owl_hip_launch(owl_queue queue, owl_kernel kernel, uint32_t gridX, uint32_t gridY, uint32_t blockX,
                    uint32_t ldsBytes, void** args);

@@ -1,6 +1,8 @@
 """Kernel-level numerics: each HIP pass against the numpy FP64 model of the same op
 (tests/fftmodel.py) on dense random digits. FFT outputs must agree to a relative
 1e-12 (FP64 with correctly rounded twiddles); rounded digits must agree exactly."""
+import re
+
 import numpy as np
 import pytest
 
@@ -12,12 +14,12 @@ pytestmark = pytest.mark.gpu
 
 # (spec, bits per word, force the runtime-plan kernels)
 # (spec, bits per word, kernel variant): "default" = shape-specialized passes where compiled
-# (one-region column pass), "colx" = the two-region column pass, "generic" = runtime-plan kernels
+# (two-region column pass), "coly" = the one-region column pass, "generic" = runtime-plan kernels
 SHAPES = [("64:2:64", 12.0, "default"), ("256:2:256", 15.0, "default"), ("256:2:256", 15.0, "generic"),
-          ("256:2:256", 15.0, "colx"), ("256:3:256", 8.0, "default"), ("256:3:256", 8.0, "colx"),
-          ("1K:3:256", 8.0, "default"), ("1K:3:256", 8.0, "generic"), ("1K:3:256", 8.0, "colx"),
+          ("256:2:256", 15.0, "coly"), ("256:3:256", 8.0, "default"), ("256:3:256", 8.0, "coly"),
+          ("1K:3:256", 8.0, "default"), ("1K:3:256", 8.0, "generic"), ("1K:3:256", 8.0, "coly"),
           ("256:13:256", 8.0, "default"), ("512:9:256", 8.0, "default"), ("1K:15:256", 17.6, "default"),
-          ("1K:15:256", 17.6, "generic"), ("1K:15:256", 17.6, "colx")]
+          ("1K:15:256", 17.6, "generic"), ("1K:15:256", 17.6, "coly")]
 
 
 def rel(a, b):
@@ -27,7 +29,7 @@ def rel(a, b):
 def setup(spec, bpw, variant, seed=1):
     N = owl.best_fit(1000, spec)[1]
     E = workloads.next_prime(int(bpw * N))
-    g = owl.Gpu(E, spec, generic=variant == "generic", col_two_region=variant == "colx")
+    g = owl.Gpu(E, spec, generic=variant == "generic", col_one_region=variant == "coly")
     m = Model(E, g.W, g.H, g.R)
     return g, m, np.random.default_rng(seed)
 
@@ -84,16 +86,15 @@ def test_kernel_selection():
     # the 7.5M-word shape runs the shape-specialized passes; generic=True forces the runtime-plan ones
     E = workloads.next_prime(139_000_000)
     with owl.Gpu(E) as g:
-        assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "coly3840")
-    with owl.Gpu(E, col_two_region=True) as g:
         assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840")
+    with owl.Gpu(E, col_one_region=True) as g:
+        assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "coly3840")
     with owl.Gpu(E, generic=True) as g:
-        assert g.row_kernel.startswith("row") and not g.row_kernel.startswith("rowx")
-        assert g.col_kernel.startswith("col") and not g.col_kernel.startswith("colx")
+        assert re.fullmatch(r"row\d+", g.row_kernel) and re.fullmatch(r"col\d+", g.col_kernel)
 
 
-@pytest.mark.parametrize("spec,bpw,variant", [("1K:3:256", 12.0, "default"), ("1K:3:256", 12.0, "colx"),
-                                              ("1K:15:256", 17.6, "default"), ("1K:15:256", 17.6, "colx"),
+@pytest.mark.parametrize("spec,bpw,variant", [("1K:3:256", 12.0, "default"), ("1K:3:256", 12.0, "coly"),
+                                              ("1K:15:256", 17.6, "default"), ("1K:15:256", 17.6, "coly"),
                                               ("1K:15:256", 17.6, "generic")])
 def test_mul_pass(spec, bpw, variant):
     # COL_FWD on a second operand, then COL_MUL, against the model's spectrum product

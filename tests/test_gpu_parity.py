@@ -60,13 +60,13 @@ def test_ll_res64_vs_gmp(spec, bpw):
         assert out.res64 == res[-1]
 
 
-VARIANTS = {"default": {}, "generic": {"generic": True}, "colx": {"col_two_region": True}}
+VARIANTS = {"default": {}, "generic": {"generic": True}, "coly": {"col_one_region": True}}
 
 
 @pytest.mark.parametrize("spec,bpw,variant", [("64:2:64", 12.0, "default"), ("256:2:256", 15.0, "default"),
-                                              ("256:2:256", 15.0, "generic"), ("256:2:256", 15.0, "colx"),
+                                              ("256:2:256", 15.0, "generic"), ("256:2:256", 15.0, "coly"),
                                               ("256:13:256", 8.0, "default"), ("1K:3:256", 12.0, "default"),
-                                              ("1K:3:256", 12.0, "generic"), ("1K:3:256", 12.0, "colx")])
+                                              ("1K:3:256", 12.0, "generic"), ("1K:3:256", 12.0, "coly")])
 def test_full_residue_vs_gmp(spec, bpw, variant):
     E, N = exponent_for(spec, bpw)
     announce("residue", spec, E, variant)
@@ -163,7 +163,7 @@ def test_config_a_golden(golden):
         assert out.nErrors == 0 and out.maxRoe < 0.4
 
 
-@pytest.mark.parametrize("variant", ["default", "colx", "generic"])
+@pytest.mark.parametrize("variant", ["default", "coly", "generic"])
 def test_config_b_golden_short(golden, variant):
     head, res = golden("B_prp_short")
     E = head["E"]

@@ -114,3 +114,10 @@ def test_tune_table_round_trip(tmp_path):
     with pytest.raises(owl.OwlError):
         owl.tune_load(bad)
     owl.tune_clear()
+
+
+def test_kernel_names_cover_the_specialized_passes():
+    names = gpuowl_amd.kernel_names()
+    assert len(names) == len(set(names))
+    for n in ("rowx1024r4", "rowx256r2", "colx3840", "coly3840", "coly512", "carryB", "readResidue"):
+        assert n in names
