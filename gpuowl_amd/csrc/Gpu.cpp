@@ -207,11 +207,13 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   for (int i = 0; i < 4; ++i) { J.words[i] = alloc<int2>(ND); }
   for (int i = 0; i < 2; ++i) { J.carry[i] = alloc<int64_t>(size_t(nb) * W); }
   J.stair = alloc<uint64_t>(size_t(nb) * W);
+  J.ticket = alloc<uint64_t>(1);
   J.roe = alloc<uint32_t>(OWL_ROE_SIZE);
   J.flags = alloc<uint32_t>(8);
   J.small = alloc<int32_t>(64);
   J.sum = alloc<uint64_t>(1);
   owlCheck(owl_hip_fill32(queue_, J.stair, 0, 2 * nb * W), "fill");
+  owlCheck(owl_hip_fill32(queue_, J.ticket, 0, 2), "fill");
   owlCheck(owl_hip_fill32(queue_, J.roe, 0, OWL_ROE_SIZE), "fill");
   owlCheck(owl_hip_fill32(queue_, J.flags, 0, 8), "fill");
   for (int i = 0; i < 4; ++i) { owlCheck(owl_hip_fill32(queue_, J.words[i], 0, 2 * ND), "fill"); }

@@ -155,68 +155,75 @@ void Server::fused(Group& g, u32 count, u32 roeSlot) {
   }
 }
 
-void Server::runRound(Group& g) {
+// `blocks` Gerbicz blocks of blockSize iterations for every member of the group
+void Server::runRound(Group& g, u32 blocks) {
   Gpu& L = *g.lead;
   const u32 B = opt_.blockSize;
   Gpu::Launch ln = g.launch();
+  std::vector<u32> kStart;
+  for (Job* j : g.members) { kStart.push_back(j->k); }
 
-  // Gerbicz block update: check *= data (src/Gpu.cpp:1466-1471)
-  if (!g.ll) {
-    std::vector<Job*> mul;
+  for (u32 blk = 0; blk < blocks; ++blk) {
+    // Gerbicz block update: check *= data (src/Gpu.cpp:1466-1471); after the first block
+    // every member takes it, so only the first may need a subset launch
+    if (!g.ll) {
+      std::vector<Job*> mul;
+      for (Job* j : g.members) {
+        if (j->skip) {
+          j->skip = false;
+        } else {
+          mul.push_back(j);
+        }
+      }
+      if (!mul.empty()) { L.modMul(Gpu::CHECK, Gpu::DATA, false, subset(g, mul)); }
+    }
+
+    // segment ends: the requested iteration of every job that reaches it in this block
+    std::vector<u32> cuts;
     for (Job* j : g.members) {
-      if (j->skip) {
-        j->skip = false;
+      if (!j->captured && j->req.iters > j->k && j->req.iters <= j->k + B) { cuts.push_back(j->req.iters - j->k); }
+    }
+    cuts.push_back(B);
+    std::sort(cuts.begin(), cuts.end());
+    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+
+    // round-off is measured (slot 0) in the first two blocks of every job, then not
+    bool measure = false;
+    for (Job* j : g.members) { measure = measure || j->k < 2 * B; }
+    const u32 roeSlot = measure ? 0 : OWL_ROE_SIZE;
+    u32 pos = 0;
+    for (u32 cut : cuts) {
+      const u32 n = cut - pos;
+      if (L.useLongCarry) {
+        for (u32 i = 0; i < n; ++i) { L.square(Gpu::DATA, Gpu::DATA, true, true, false, g.ll, ln); }
       } else {
-        mul.push_back(j);
+        L.rowLeadInTo(Gpu::DATA, 0, ln);
+        fused(g, n - 1, roeSlot);
+        L.colSquare(ln);
+        L.rowCarryOut(Gpu::DATA, g.ll, false, roeSlot, ln);
+        L.carryAll(Gpu::DATA, ln);
       }
-    }
-    if (!mul.empty()) {
-      Gpu::Launch ls = subset(g, mul);
-      L.modMul(Gpu::CHECK, Gpu::DATA, false, ls);
-    }
-  }
-
-  // segment ends: the requested iteration of every job that reaches it in this round
-  std::vector<u32> cuts;
-  for (Job* j : g.members) {
-    if (!j->captured && j->req.iters > j->k && j->req.iters <= j->k + B) { cuts.push_back(j->req.iters - j->k); }
-  }
-  cuts.push_back(B);
-  std::sort(cuts.begin(), cuts.end());
-  cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
-
-  // round-off is measured (slot 0) in the first two rounds of every job, then not
-  bool measure = false;
-  for (Job* j : g.members) { measure = measure || j->k < 2 * B; }
-  const u32 roeSlot = measure ? 0 : OWL_ROE_SIZE;
-  u32 pos = 0;
-  for (u32 cut : cuts) {
-    const u32 n = cut - pos;
-    if (L.useLongCarry) {
-      for (u32 i = 0; i < n; ++i) { L.square(Gpu::DATA, Gpu::DATA, true, true, false, g.ll, ln); }
-    } else {
-      L.rowLeadInTo(Gpu::DATA, 0, ln);
-      fused(g, n - 1, roeSlot);
-      L.colSquare(ln);
-      L.rowCarryOut(Gpu::DATA, g.ll, false, roeSlot, ln);
-      L.carryAll(Gpu::DATA, ln);
+      for (Job* j : g.members) {
+        if (!j->captured && j->req.iters == j->k + cut) {
+          j->gpu->residueOn(ln.q, Gpu::DATA);
+          bool final = j->req.iters == (g.ll ? j->req.E - 2 : j->req.E);
+          if (final) { j->gpu->copyOn(ln.q, Gpu::BASE, Gpu::DATA); }
+          j->captured = true;
+        }
+      }
+      pos = cut;
     }
     for (Job* j : g.members) {
-      if (!j->captured && j->req.iters == j->k + cut) {
-        j->gpu->residueOn(ln.q, Gpu::DATA);
-        bool final = j->req.iters == (g.ll ? j->req.E - 2 : j->req.E);
-        if (final) { j->gpu->copyOn(ln.q, Gpu::BASE, Gpu::DATA); }
-        j->captured = true;
-      }
+      u32 useful = j->k < j->req.iters ? std::min(B, j->req.iters - j->k) : 0;
+      iterations_ += useful;
+      wordIterations_ += u64(useful) * j->gpu->N;
+      j->k += B;
     }
-    pos = cut;
   }
-  for (Job* j : g.members) {
-    u32 useful = j->k < j->req.iters ? std::min(B, j->req.iters - j->k) : 0;
-    iterations_ += useful;
-    wordIterations_ += u64(useful) * j->gpu->N;
-    j->k += B;
-    if (!g.ll) { j->wantCheck = (j->k % opt_.checkStep == 0) || j->k >= j->kEndEnd; }
+  // a PRP job is verified when the round crossed a check point, and past its end
+  for (size_t i = 0; i < g.members.size(); ++i) {
+    Job* j = g.members[i];
+    if (!g.ll) { j->wantCheck = (j->k / opt_.checkStep != kStart[i] / opt_.checkStep) || j->k >= j->kEndEnd; }
   }
 }
 
@@ -272,7 +279,18 @@ u32 Server::step() {
     if (g->dirty) { prepare(*g); }
     gs.push_back(g.get());
   }
-  for (Group* g : gs) { runRound(*g); }
+  // balance the round: a group of a smaller FFT runs proportionally more blocks (up to 16),
+  // never past the end of its shortest job
+  u64 nMax = 0;
+  for (Group* g : gs) { nMax = std::max<u64>(nMax, g->lead->N); }
+  for (Group* g : gs) {
+    u32 blocks = u32(std::min<u64>(16, std::max<u64>(1, nMax / g->lead->N)));
+    for (Job* j : g->members) {
+      u32 left = j->k < j->kEndEnd ? (j->kEndEnd - j->k) / opt_.blockSize : 0;
+      blocks = std::min(blocks, std::max<u32>(1, left));
+    }
+    runRound(*g, blocks);
+  }
   for (Group* g : gs) { g->lead->finish(); }
 
   // residues taken in this round

@@ -1,6 +1,8 @@
 // A request loop over one device: PRP and LL jobs arrive at any time, are admitted while
 // their device footprint fits the memory budget and a slot is free, and advance together
-// in rounds of blockSize iterations. Jobs of one FFT geometry and kind share every launch
+// in rounds of Gerbicz blocks (blockSize iterations; groups of smaller FFTs run more blocks
+// per round, so that a round takes about as long for every group). Jobs of one FFT
+// geometry and kind share every launch
 // of a round (one group, one queue, one captured graph of fused squarings); groups of
 // different geometries run on their own queues and overlap on the device. Admission,
 // retirement and Gerbicz verification happen between rounds, so a request that arrives
@@ -107,7 +109,7 @@ private:
   void admit();
   void prepare(Group& g);
   Gpu::Launch subset(Group& g, const std::vector<Job*>& js);
-  void runRound(Group& g);
+  void runRound(Group& g, u32 blocks);
   void fused(Group& g, u32 count, u32 roeSlot);
   void verify(Group& g);
   void retire();

@@ -63,6 +63,7 @@ struct JobDesc {
   int2* words[4];         // [0] data, [1] aux, [2] check, [3] base: integer words (line-block layout)
   int64_t* carry[2];      // carry shuttle (nb*W) and carry-propagation ping-pong
   uint64_t* stair;        // nb*W tagged carries of the fused row pass (tag << 48 | 48-bit carry)
+  uint64_t* ticket;       // fused row-pass start counter: every launch takes nb+1 tickets
   uint32_t* roe;          // OWL_ROE_SIZE max-round-off slots (float bits)
   uint32_t* flags;        // [0] carry leftover, [1] compare mismatch
   int32_t* small;         // 64 ints for the residue read

@@ -259,12 +259,27 @@ __device__ __forceinline__ uint64_t pollCarry(const JobDesc& J, uint64_t* p, uin
   return v;
 }
 
+// The fused row pass orders its groups by start, not by blockIdx: each group takes the
+// next ticket of the job's counter, and ticket mod (nb + 1) is its block. A group then
+// only waits for groups that have already started, so the carry hand-off progresses even
+// when other kernels (other queues) hold the rest of the device. Every fused launch takes
+// exactly nb + 1 tickets, so the 64-bit counter never needs a reset.
+__device__ __forceinline__ int fusedBlock(const JobDesc& J, int* slot) {
+  if (threadIdx.x == 0) { *slot = int(atomicAdd((unsigned long long*)J.ticket, 1ull) % (unsigned long long)(J.nb + 1)); }
+  __syncthreads();
+  return *slot;
+}
+
 // N polls issued together: every carry's first load is in flight at once, then only the
 // ones whose tag is not current yet are loaded again.
 template<int N>
-__device__ __forceinline__ void pollCarries(const JobDesc& J, uint64_t* const* p, uint32_t tag, uint64_t* v) {
+__device__ __forceinline__ void pollStart(uint64_t* const* p, uint64_t* v) {
 #pragma unroll
   for (int k = 0; k < N; ++k) { v[k] = __hip_atomic_load(p[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+}
+// Continue polling from values v already loaded by pollStart (independent work can run between).
+template<int N>
+__device__ __forceinline__ void pollFinish(const JobDesc& J, uint64_t* const* p, uint32_t tag, uint64_t* v) {
   uint32_t n = 0;
   while (true) {
     bool ready = true;
@@ -281,6 +296,11 @@ __device__ __forceinline__ void pollCarries(const JobDesc& J, uint64_t* const* p
       if ((uint32_t)(v[k] >> 48) != tag) { v[k] = __hip_atomic_load(p[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
     }
   }
+}
+template<int N>
+__device__ __forceinline__ void pollCarries(const JobDesc& J, uint64_t* const* p, uint32_t tag, uint64_t* v) {
+  pollStart<N>(p, v);
+  pollFinish<N>(J, p, tag, v);
 }
 
 // Round-off of a workgroup: each wave's max goes to part[wave]; after a barrier, thread 0

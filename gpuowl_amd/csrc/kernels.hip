@@ -113,7 +113,8 @@ __global__ __launch_bounds__(T) void k_row(const JobDesc* __restrict__ jobs, int
   extern __shared__ T2 lds[];
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, nb = J.nb;
-  const int g = blockIdx.x;
+  __shared__ int ticketSlot;
+  const int g = (mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x;
   const int lb = (g == nb) ? 0 : g;
   const int tid = threadIdx.x;
   const int LW = R * W;

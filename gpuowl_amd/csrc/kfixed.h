@@ -432,7 +432,8 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   float* roePart = reinterpret_cast<float*>(lds + R * REG);  // one float per wave, after the line regions
   const JobDesc& J = jobs[blockIdx.y];
   const int nb = J.nb;
-  const int g = blockIdx.x;
+  __shared__ int ticketSlot;
+  const int g = (mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x;
   const int lb = (g == nb) ? 0 : g;
   const int tid = threadIdx.x;
   const bool ll = opts & 1, mul3 = opts & 2;
@@ -444,6 +445,18 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   ColW cw[F0];
 #pragma unroll
   for (int k = 0; k < F0; ++k) { cw[k] = gld(J.colW + tid + k * T); }
+  double wt[R][F0][2];  // forward weights of the thread's words
+  auto weights = [&]() {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const LineW lw = gld(J.lineW + lb * R + r);
+#pragma unroll
+      for (int k = 0; k < F0; ++k) {
+        wt[r][k][0] = wordWeight(J, lw, cw[k], 0).w;
+        wt[r][k][1] = wordWeight(J, lw, cw[k], 1).w;
+      }
+    }
+  };
 
   if (mode == ROW_LEADIN) {
     const int2* src = J.words[wi] + (size_t)lb * LW;
@@ -532,7 +545,11 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       const int c = tid + k * T;
       src[k] = J.stair + (size_t)pred * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
     }
-    pollCarries<F0>(J, src, tag, got);
+    // the first poll loads go out now; the weights of the final weighting, which do not
+    // depend on the carry, are computed while they are in flight
+    pollStart<F0>(src, got);
+    weights();
+    pollFinish<F0>(J, src, tag, got);
     if (g == nb) {  // group 0 has read block 0
 #pragma unroll
       for (int k = 0; k < F0; ++k) { src[k] = J.stair + tid + k * T; }
@@ -567,15 +584,11 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   }
 
   // weight, then forward stage 0 (radix F0, Ns = 1) straight from registers
+  if (mode == ROW_LEADIN) { weights(); }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const LineW lw = gld(J.lineW + lb * R + r);
 #pragma unroll
-    for (int k = 0; k < F0; ++k) {
-      WordW w0 = wordWeight(J, lw, cw[k], 0);
-      WordW w1 = wordWeight(J, lw, cw[k], 1);
-      v[r][k] = make_double2(d[r][k].x * w0.w, d[r][k].y * w1.w);
-    }
+    for (int k = 0; k < F0; ++k) { v[r][k] = make_double2(d[r][k].x * wt[r][k][0], d[r][k].y * wt[r][k][1]); }
     Dft<F0, false>::run(v[r]);
   }
   __syncthreads();  // every thread has finished reading the inverse's LDS

@@ -56,6 +56,22 @@ def test_mix_matches_alone_and_gmp():
         assert r["tSubmit"] <= r["tAdmit"] <= r["tDone"]
 
 
+def test_rounds_balance_small_and_large_ffts():
+    # a 1.5M-word job beside 256K-word ones: the small group runs six blocks per round,
+    # with its requested iterations and check points falling inside the round
+    big = workloads.next_prime(12 * 1_572_864)
+    reqs = [(big, "prp", 400), (3_900_067, "prp", 1234), (3_900_091, "ll", 777)]
+    announce("serve balance", reqs)
+    with owl.Server(block_size=100, check_step=200) as s:
+        ids = {s.submit(E, k, n): (E, k, n) for E, k, n in reqs}
+        res = {r["id"]: r for r in s.drain()}
+    assert set(res) == set(ids)
+    for rid, (E, k, n) in ids.items():
+        r = res[rid]
+        assert r["status"] == 0 and r["nErrors"] == 0, r
+        assert r["res64"] == expected(E, k, n), (E, k, n)
+
+
 def test_admission_by_memory_budget():
     exps = workloads.primes_from(3_000_000, 4)
     spec = "256:2:256"
