@@ -206,6 +206,8 @@ def main():
                     help="requests below this exponent run to completion (the full serving config uses 10,000,000)")
     ap.add_argument("--profile", action="store_true",
                     help="only the profiled workload: K steady iterations and one state copy, no JSON line")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="only the batched config C workload: K steady batch iterations, no JSON line")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (exponents shard across GPUs "
@@ -214,6 +216,13 @@ def main():
 
     import gpuowl_amd as owl
     from gpuowl_amd import workloads
+
+    if a.profile_batch:
+        b = owl.Batch(workloads.config_c(), "1K:3:256")
+        us = b.time_squares(a.steps, a.warmup, use_graph=False)
+        print(f"profile batch C: {b.n} jobs, {a.warmup}+{a.steps} batch iterations, {us:.1f} us each")
+        b.close()
+        return
 
     E = workloads.config_b()
     g = owl.Gpu(E)
