@@ -157,7 +157,7 @@ def serve_leg(owl, workloads, count, lam, slice_iters, full_below, block=1000, c
     in [2e6, 1.4e8]: below full_below a request runs to completion (E or E - 2 iterations),
     otherwise one slice of slice_iters. Requests are submitted at their arrival time and the
     server steps rounds of `block` iterations; latency = completion - arrival."""
-    reqs = workloads.config_e(count, lam)
+    reqs = workloads.poisson_requests(count, lam)
     length = [(E - 2 if kind == "ll" else E) if E < full_below else slice_iters for _, E, kind in reqs]
     plain = PlainLoop(owl)  # measured before the served run starts
     plain_s = sum(n * plain.us_per_iter(E) * 1e-6 for n, (_, E, _) in zip(length, reqs))
@@ -196,7 +196,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--cpu-iters", type=int, default=20)
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched legs (configs C and D)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched legs")
     ap.add_argument("--batch-iters", type=int, default=100)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sample")
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -207,7 +207,7 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="only the profiled workload: K steady iterations and one state copy, no JSON line")
     ap.add_argument("--profile-batch", action="store_true",
-                    help="only the batched config C workload: K steady batch iterations, no JSON line")
+                    help="only the batched workload of 64 primes >= 20M at 1K:3:256: K steady batch iterations, no JSON line")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (exponents shard across GPUs "
@@ -218,13 +218,13 @@ def main():
     from gpuowl_amd import workloads
 
     if a.profile_batch:
-        b = owl.Batch(workloads.config_c(), "1K:3:256")
+        b = owl.Batch(workloads.uniform64_20m(), "1K:3:256")
         us = b.time_squares(a.steps, a.warmup, use_graph=False)
-        print(f"profile batch C: {b.n} jobs, {a.warmup}+{a.steps} batch iterations, {us:.1f} us each")
+        print(f"profile batch: {b.n} jobs, {a.warmup}+{a.steps} batch iterations, {us:.1f} us each")
         b.close()
         return
 
-    E = workloads.config_b()
+    E = workloads.p139m()
     g = owl.Gpu(E)
     N = g.N
 
@@ -237,7 +237,7 @@ def main():
 
     # parity: res64 against the GMP golden, before timing
     parity = None
-    gold = golden_res("B_prp_short")
+    gold = golden_res("p139000003_prp_short")
     if gold:
         out, res = g.prp(200, block_size=100, check_step=100, res_step=100)
         parity = {"k": [100, 200], "ok": res == [gold[100], gold[200]], "max_roe": out.maxRoe,
@@ -290,8 +290,8 @@ def main():
     batched = None
     if not a.no_batch:
         batched = {
-            "C_uniform_64x1K:3:256": leg(batch_leg, owl, workloads.config_c(), "1K:3:256", a.batch_iters, 10),
-            "D_uneven_200": leg(batch_leg, owl, workloads.config_d(), "", a.batch_iters, 10),
+            "uniform_64x20M_at_1K:3:256": leg(batch_leg, owl, workloads.uniform64_20m(), "1K:3:256", a.batch_iters, 10),
+            "uneven_200_exponents_1M_to_60M": leg(batch_leg, owl, workloads.uneven200(), "", a.batch_iters, 10),
         }
 
     served = None

@@ -1,13 +1,14 @@
-"""Synthetic workloads of the benchmark configurations (exponents only; every
-residue starts from 3 for PRP or 4 for LL).
+"""Synthetic benchmark workloads (exponents only; every residue starts from 3 for PRP or
+4 for LL).
 
-  A  p = smallest prime >= 4,000,000 at the reference's FFT choice (256:2:256)
-  B  p = smallest prime >= 139,000,000 at 1K:15:256 (N = 7,864,320)
-  C  the first 64 primes >= 20,000,000, all at 1K:3:256 (N = 1,572,864)
-  D  200 primes log-uniform in [1e6, 6e7]: splitmix64(seed 1234), u = (x >> 11) * 2^-53,
-     e = floor(exp(ln 1e6 + u ln 60)), p = nextprime(e), redrawn on a duplicate
-  E  serving arrivals: splitmix64(seed 7); draws alternate exponent u (log-uniform
-     prime in [2e6, 1.4e8]), kind u (PRP if < 0.5 else LL), inter-arrival -ln(1-u)/lambda
+  p4m()           p = smallest prime >= 4,000,000 at the reference's FFT choice (256:2:256)
+  p139m()         p = smallest prime >= 139,000,000 at 1K:15:256 (N = 7,864,320)
+  uniform64_20m() the first 64 primes >= 20,000,000, all at 1K:3:256 (N = 1,572,864)
+  uneven200()     200 primes log-uniform in [1e6, 6e7]: splitmix64(seed 1234), u = (x >> 11) * 2^-53,
+                  e = floor(exp(ln 1e6 + u ln 60)), p = nextprime(e), redrawn on a duplicate
+  poisson_requests(count, lam)
+                  serving arrivals: splitmix64(seed 7); draws alternate exponent u (log-uniform
+                  prime in [2e6, 1.4e8]), kind u (PRP if < 0.5 else LL), inter-arrival -ln(1-u)/lambda
 """
 import math
 
@@ -63,19 +64,19 @@ def primes_from(n, count):
     return out
 
 
-def config_a():
+def p4m():
     return next_prime(4_000_000)
 
 
-def config_b():
+def p139m():
     return next_prime(139_000_000)
 
 
-def config_c():
+def uniform64_20m():
     return primes_from(20_000_000, 64)
 
 
-def config_d(count=200, seed=1234, lo=1e6, span=60.0):
+def uneven200(count=200, seed=1234, lo=1e6, span=60.0):
     g = splitmix64(seed)
     out, seen = [], set()
     while len(out) < count:
@@ -88,7 +89,7 @@ def config_d(count=200, seed=1234, lo=1e6, span=60.0):
     return out
 
 
-def config_e(count, lam, seed=7, lo=2e6, hi=1.4e8):
+def poisson_requests(count, lam, seed=7, lo=2e6, hi=1.4e8):
     """[(arrival_time_s, exponent, kind)] for count Poisson arrivals at rate lam per second."""
     g = splitmix64(seed)
     t, out = 0.0, []

@@ -4,10 +4,10 @@
 Each fixture res_<name>.txt holds a header line and "k res64" lines:
     # E=<E> kind=<prp|ll> iters=<n> step=<s>
     1000 0123456789abcdef
-The exponents are the workloads of the benchmark configurations:
-  A  p = 4,000,037 (smallest prime >= 4,000,000), PRP-3
-  B  p = 139,000,003 (smallest prime >= 139,000,000), PRP-3 and LL
-  C  the first 64 primes >= 20,000,000, PRP-3
+Fixtures are named by exponent and kind:
+  p4000037_prp            100,000 PRP-3 iterations of p = 4,000,037 (smallest prime >= 4,000,000)
+  p139000003_{prp,ll}     20,000 PRP-3 / LL iterations of p = 139,000,003, and *_short: 200
+  p<E>_prp                1,000 PRP-3 iterations of each of the first 64 primes >= 20,000,000
 Usage: python tests/golden/make_goldens.py [--jobs N] [names...]"""
 import argparse, os, subprocess, sys, concurrent.futures as cf
 
@@ -59,13 +59,13 @@ def jobs():
     J = {}
     A = next_prime(4_000_000)
     B = next_prime(139_000_000)
-    J["A_prp"] = (A, 0, 100_000, 1000)
-    J["B_prp"] = (B, 0, 20_000, 1000)
-    J["B_ll"] = (B, 1, 20_000, 1000)
-    J["B_prp_short"] = (B, 0, 200, 100)
-    J["B_ll_short"] = (B, 1, 200, 100)
-    for i, p in enumerate(primes_from(20_000_000, 64)):
-        J[f"C{i:02d}_prp"] = (p, 0, 1000, 500)
+    J[f"p{A}_prp"] = (A, 0, 100_000, 1000)
+    J[f"p{B}_prp"] = (B, 0, 20_000, 1000)
+    J[f"p{B}_ll"] = (B, 1, 20_000, 1000)
+    J[f"p{B}_prp_short"] = (B, 0, 200, 100)
+    J[f"p{B}_ll_short"] = (B, 1, 200, 100)
+    for p in primes_from(20_000_000, 64):
+        J[f"p{p}_prp"] = (p, 0, 1000, 500)
     return J
 
 

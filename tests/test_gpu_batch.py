@@ -84,14 +84,14 @@ def load_golden(name):
     return res
 
 
-def test_config_c_batch_vs_goldens():
+def test_uniform_batch_64x20m_vs_goldens():
     # the first 64 primes >= 20,000,000, all at 1K:3:256, advanced together
-    exps = workloads.config_c()
-    gold = [load_golden(f"C{i:02d}_prp") for i in range(64)]
+    exps = workloads.uniform64_20m()
+    gold = [load_golden(f"p{p}_prp") for p in exps]
     have = [i for i, g in enumerate(gold) if g]
     if not have:
-        pytest.skip("config C goldens not generated")
-    announce("config C batch", len(exps))
+        pytest.skip("goldens of the 64 primes >= 20M not generated")
+    announce("64-prime batch at 1K:3:256", len(exps))
     with owl.Batch(exps, "1K:3:256") as b:
         assert b.n_groups == 1 and b.job_info(0)["N"] == 1_572_864
         out, rows = b.prp(1000, block_size=500, check_step=500, res_step=500)

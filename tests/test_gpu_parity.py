@@ -152,10 +152,10 @@ def test_is_equal_and_copy():
         assert not g.is_equal(owl.DATA, owl.AUX)
 
 
-def test_config_a_golden(golden):
-    head, res = golden("A_prp")
+def test_p4000037_prp_golden(golden):
+    head, res = golden("p4000037_prp")
     E = head["E"]
-    announce("config A", E)
+    announce("p = 4,000,037 golden", E)
     with owl.Gpu(E) as g:
         assert g.spec == "256:2:256:101"
         out, got = g.prp(10_000, block_size=1000, check_step=5000, res_step=1000)
@@ -164,10 +164,10 @@ def test_config_a_golden(golden):
 
 
 @pytest.mark.parametrize("variant", ["default", "coly", "generic"])
-def test_config_b_golden_short(golden, variant):
-    head, res = golden("B_prp_short")
+def test_p139000003_prp_golden_short(golden, variant):
+    head, res = golden("p139000003_prp_short")
     E = head["E"]
-    announce("config B prp", E, variant)
+    announce("p = 139,000,003 prp golden", E, variant)
     with owl.Gpu(E, **VARIANTS[variant]) as g:
         assert g.spec == "1K:15:256:101" and g.N == 7_864_320
         out, got = g.prp(200, block_size=100, check_step=100, res_step=100)
@@ -175,10 +175,10 @@ def test_config_b_golden_short(golden, variant):
         assert out.nErrors == 0 and out.maxRoe < 0.4
 
 
-def test_config_b_ll_golden_short(golden):
-    head, res = golden("B_ll_short")
+def test_p139000003_ll_golden_short(golden):
+    head, res = golden("p139000003_ll_short")
     E = head["E"]
-    announce("config B ll", E)
+    announce("p = 139,000,003 ll golden", E)
     with owl.Gpu(E) as g:
         out, got = g.ll(200, res_step=100)
         assert got == [res[100], res[200]]

@@ -38,8 +38,8 @@ def test_no_fallback_when_library_missing(monkeypatch, tmp_path):
 
 
 @pytest.mark.parametrize("E,spec,N", [
-    (4_000_037, "256:2:256:101", 262144),        # config A
-    (139_000_003, "1K:15:256:101", 7864320),      # config B, maxExp ~142.56M
+    (4_000_037, "256:2:256:101", 262144),        # p = 4,000,037: the reference picks 256:2:256
+    (139_000_003, "1K:15:256:101", 7864320),      # p = 139,000,003: 1K:15:256, maxExp ~142.56M
     (1_029_689, "256:2:256:101", 262144),
     (56_430_277, "1K:6:256:101", 3145728),
 ])
@@ -49,13 +49,13 @@ def test_best_fit_matches_reference_choice(E, spec, N):
     assert mx >= E
 
 
-def test_best_fit_config_b_max_exp():
+def test_best_fit_p139000003_max_exp():
     s, n, mx = gpuowl_amd.best_fit(139_000_003)
     assert abs(mx - 142_560_000) < 10_000
 
 
-def test_best_fit_config_d_distribution():
-    ps = workloads.config_d()
+def test_best_fit_uneven200_distribution():
+    ps = workloads.uneven200()
     assert ps[:5] == [19_917_551, 11_330_491, 2_287_861, 3_503_053, 19_297_693]
     assert min(ps) == 1_029_689 and max(ps) == 56_430_277
     Ns = [gpuowl_amd.best_fit(p)[1] for p in ps]
@@ -74,9 +74,9 @@ def test_spec_parsing_and_errors():
 
 
 def test_workloads():
-    assert workloads.config_a() == 4_000_037
-    assert workloads.config_b() == 139_000_003
-    c = workloads.config_c()
+    assert workloads.p4m() == 4_000_037
+    assert workloads.p139m() == 139_000_003
+    c = workloads.uniform64_20m()
     assert len(c) == 64 and c[0] == 20_000_003 and c[-1] == 20_000_879
 
 
