@@ -184,6 +184,27 @@ def test_p139000003_ll_golden_short(golden):
         assert got == [res[100], res[200]]
 
 
+def test_p139000003_prp_golden_20000(golden):
+    # the full stream: 20,000 PRP-3 iterations at 7.5M words, res64 every 1,000, Gerbicz
+    # checks every 5,000, against GMP
+    head, res = golden("p139000003_prp")
+    E = head["E"]
+    announce("p = 139,000,003 prp golden, 20,000 iterations", E)
+    with owl.Gpu(E) as g:
+        out, got = g.prp(20_000, block_size=1000, check_step=5000, res_step=1000)
+    assert got == [res[k] for k in range(1000, 20_001, 1000)]
+    assert out.nErrors == 0 and out.nChecks == 4 and out.maxRoe < 0.4
+
+
+def test_p139000003_ll_golden_20000(golden):
+    head, res = golden("p139000003_ll")
+    E = head["E"]
+    announce("p = 139,000,003 ll golden, 20,000 iterations", E)
+    with owl.Gpu(E) as g:
+        out, got = g.ll(20_000, res_step=1000)
+    assert got == [res[k] for k in range(1000, 20_001, 1000)]
+
+
 @pytest.mark.parametrize("spec,bpw", [("64:2:64", 3.0), ("256:3:256", 3.2), ("256:3:256", 8.0), ("1K:3:256", 5.0)])
 def test_low_bits_per_word_stays_exact(spec, bpw):
     # the fused carry must stay stable (no growth of the hand-off word) at low bits per word
