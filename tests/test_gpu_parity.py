@@ -158,9 +158,9 @@ def test_p4000037_prp_golden(golden):
     announce("p = 4,000,037 golden", E)
     with owl.Gpu(E) as g:
         assert g.spec == "256:2:256:101"
-        out, got = g.prp(10_000, block_size=1000, check_step=5000, res_step=1000)
-        assert got == [res[k] for k in range(1000, 10_001, 1000)]
-        assert out.nErrors == 0 and out.maxRoe < 0.4
+        out, got = g.prp(100_000, block_size=1000, check_step=10_000, res_step=1000)
+    assert got == [res[k] for k in range(1000, 100_001, 1000)]
+    assert out.nErrors == 0 and out.nChecks == 10 and out.maxRoe < 0.4
 
 
 @pytest.mark.parametrize("variant", ["default", "coly", "generic"])
