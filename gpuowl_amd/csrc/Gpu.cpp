@@ -54,11 +54,12 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   if (W * H != ND) { throw std::invalid_argument("bad FFT geometry"); }
   // a tuned choice for this FFT length applies when the caller chose nothing
   TuneChoice tuned;
-  if (!opt.width && !opt.lineBlock && !opt.generic && !opt.colOneRegion && TuneTable::get(N, &tuned)) {
+  if (!opt.width && !opt.lineBlock && !opt.generic && !opt.colOneRegion && !opt.colOnePair && TuneTable::get(N, &tuned)) {
     opt.width = tuned.width;
     opt.lineBlock = tuned.lineBlock;
     opt.generic = tuned.generic;
     opt.colOneRegion = tuned.colOneRegion;
+    opt.colOnePair = tuned.colOnePair;
     tunedLineBlock_ = true;
   }
   if (opt.width) {
@@ -135,6 +136,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   }
   rowLds = R * W * 16;
   colLds = 2 * H * 16;
+  colGroups = W / 2;
   if (rowFixed) {
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
@@ -144,10 +146,24 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     rowKernel = "row" + std::to_string(rowT);
   }
   if (colFixed) {
+    u32 tables = 1;
+    // Two column pairs per group ("colx<H>p2", compiled where one k_colx group fills a CU's
+    // LDS) when W/4 groups still give every CU one: the second pair's loads then overlap the
+    // first pair's work. One more twiddle table in LDS.
+    if (useX && !opt.colOnePair && haveKernel(colX + "p2")) {
+      int cus = 0;
+      owlCheck(owl_hip_device_info(opt.device, nullptr, 0, nullptr, nullptr, &cus, nullptr), "device info");
+      if (W % 4 == 0 && cus > 0 && W / 4 >= u32(cus)) {
+        colKernel = colX + "p2";
+        colGroups = W / 4;
+        tables = 2;
+      }
+    }
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
-    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + 416) * 16;  // colRegion<H>() regions + COLTW_SIZE (kfixed.h)
+    // colRegion<H>() regions + COLTW_SIZE-entry tables (kfixed.h)
+    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + 416 * tables) * 16;
   } else {
     colKernel = "col" + std::to_string(colT);
   }
@@ -347,17 +363,17 @@ void Gpu::carryAll(Buf b, Launch L) {
 
 void Gpu::colSquare(Launch L) {
   int mode = COL_SQUARE, fi = 0, oi = 0;
-  kCol(L.q, W / 2, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
+  kCol(L.q, colGroups, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
 }
 
 void Gpu::colMul(Launch L) {
   int mode = COL_MUL, fi = 0, oi = 2;
-  kCol(L.q, W / 2, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
+  kCol(L.q, colGroups, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
 }
 
 void Gpu::colFwdSpec(Launch L) {
   int mode = COL_FWD, fi = 2, oi = 2;
-  kCol(L.q, W / 2, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
+  kCol(L.q, colGroups, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
 }
 
 // ---------------------------------------------------------------- squaring

@@ -39,6 +39,7 @@ struct GpuOptions {
   bool longCarry = false;  // lead-out + lead-in around every iteration (the reference's -carry long)
   bool generic = false;    // runtime-plan kernels even where a shape-specialized pass exists
   bool colOneRegion = false;  // the one-region column pass (k_coly) where k_colx exists
+  bool colOnePair = false;    // k_colx with one column pair per group where two would be chosen
   u32 width = 0;           // W of the W x H split of the FFT; 0 = the reference's (refactored when too tall)
   u32 lineBlock = 0;       // lines per row-pass block; 0 = automatic
   int device = 0;
@@ -74,6 +75,7 @@ public:
   const u32 E;
   const FFTConfig fft;
   u32 N, ND, W, H, R, nb, rowT, colT, rowLds, colLds;
+  u32 colGroups;  // column-pass groups per job (W/2 = one pair each)
   std::string rowKernel, colKernel;
   bool useLongCarry;
   std::vector<u32> rowPlan, colPlan;

@@ -17,7 +17,7 @@ std::map<u32, TuneChoice>& table() {
   static std::map<u32, TuneChoice> t;
   return t;
 }
-int flagsOf(const TuneChoice& c) { return (c.generic ? 2 : 0) | (c.colOneRegion ? 4 : 0); }
+int flagsOf(const TuneChoice& c) { return (c.generic ? 2 : 0) | (c.colOneRegion ? 4 : 0) | (c.colOnePair ? 8 : 0); }
 }  // namespace
 
 void TuneTable::set(u32 N, const TuneChoice& c) {
@@ -54,6 +54,7 @@ void TuneTable::load(const std::string& path) {
     if (!(is >> c.width >> c.lineBlock >> flags >> c.us)) { throw std::invalid_argument("bad tune line: " + line); }
     c.generic = flags & 2;
     c.colOneRegion = flags & 4;
+    c.colOnePair = flags & 8;
     got[N] = c;
   }
   std::lock_guard<std::mutex> lock(mtx);
@@ -82,12 +83,13 @@ std::vector<TuneResult> tuneShape(u32 E, const std::string& spec, u32 iters, u32
     if (h < 64 || h > 4096 || fftPlan(w).empty() || fftPlan(h).empty()) { continue; }
     for (u32 r = 1; r <= 16 && r * w <= 4096; r *= 2) {
       if (h % r) { continue; }
-      for (int v = 0; v < 3; ++v) {
+      for (int v = 0; v < 4; ++v) {
         TuneChoice c;
         c.width = w;
         c.lineBlock = r;
         c.generic = v == 2;
         c.colOneRegion = v == 1;
+        c.colOnePair = v == 3;
         cands.push_back(c);
       }
     }
@@ -101,6 +103,7 @@ std::vector<TuneResult> tuneShape(u32 E, const std::string& spec, u32 iters, u32
     opt.lineBlock = c.lineBlock;
     opt.generic = c.generic;
     opt.colOneRegion = c.colOneRegion;
+    opt.colOnePair = c.colOnePair;
     try {
       Gpu g(E, fft, opt);
       auto key = std::make_tuple(g.rowKernel, g.colKernel, g.W, g.R, g.useLongCarry);

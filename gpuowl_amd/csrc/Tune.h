@@ -12,7 +12,7 @@
 struct TuneChoice {
   u32 width = 0;      // W of the W x H split (0 = the engine's rule)
   u32 lineBlock = 0;  // R (0 = the engine's rule)
-  bool generic = false, colOneRegion = false;
+  bool generic = false, colOneRegion = false, colOnePair = false;
   double us = 0;      // measured microseconds per iteration
 };
 
@@ -21,7 +21,8 @@ namespace TuneTable {
 void set(u32 N, const TuneChoice& c);
 bool get(u32 N, TuneChoice* c);
 void clear();
-// text lines "N width lineBlock flags us" (flags: 2 = generic, 4 = one-region column pass)
+// text lines "N width lineBlock flags us" (flags: 2 = generic, 4 = one-region column pass,
+// 8 = one column pair per column group)
 void load(const std::string& path);
 void save(const std::string& path);
 }  // namespace TuneTable

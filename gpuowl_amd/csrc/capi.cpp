@@ -70,6 +70,7 @@ int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, 
     opt.longCarry = flags & OWL_LONG_CARRY;
     opt.generic = flags & OWL_GENERIC;
     opt.colOneRegion = flags & OWL_COL_ONE_REGION;
+    opt.colOnePair = flags & OWL_COL_ONE_PAIR;
     opt.lineBlock = lineBlock;
     *out = new owl_gpu_s{new Gpu(E, fft, opt)};
   });
@@ -279,7 +280,8 @@ int owl_tune_run(uint32_t E, const char* spec, uint32_t iters, owl_tune_entry_t*
       owl_tune_entry_t& o = out[i];
       o.width = r.choice.width;
       o.lineBlock = r.choice.lineBlock;
-      o.flags = (r.choice.generic ? OWL_GENERIC : 0) | (r.choice.colOneRegion ? OWL_COL_ONE_REGION : 0);
+      o.flags = (r.choice.generic ? OWL_GENERIC : 0) | (r.choice.colOneRegion ? OWL_COL_ONE_REGION : 0) |
+                (r.choice.colOnePair ? OWL_COL_ONE_PAIR : 0);
       o.usPerIter = r.choice.us;
       o.W = r.W, o.H = r.H, o.R = r.R;
       o.longCarry = r.longCarry;
@@ -297,6 +299,7 @@ int owl_tune_set(uint32_t N, uint32_t width, uint32_t lineBlock, int flags, doub
     c.lineBlock = lineBlock;
     c.generic = flags & OWL_GENERIC;
     c.colOneRegion = flags & OWL_COL_ONE_REGION;
+    c.colOnePair = flags & OWL_COL_ONE_PAIR;
     c.us = us;
     TuneTable::set(N, c);
   });
@@ -308,7 +311,7 @@ int owl_tune_get(uint32_t N, uint32_t* width, uint32_t* lineBlock, int* flags, d
     *found = TuneTable::get(N, &c);
     *width = c.width;
     *lineBlock = c.lineBlock;
-    *flags = (c.generic ? OWL_GENERIC : 0) | (c.colOneRegion ? OWL_COL_ONE_REGION : 0);
+    *flags = (c.generic ? OWL_GENERIC : 0) | (c.colOneRegion ? OWL_COL_ONE_REGION : 0) | (c.colOnePair ? OWL_COL_ONE_PAIR : 0);
     *us = c.us;
   });
 }
@@ -338,6 +341,7 @@ int owl_server_create(uint64_t memBudget, uint32_t maxJobs, uint32_t blockSize, 
     o.gpu.longCarry = flags & OWL_LONG_CARRY;
     o.gpu.generic = flags & OWL_GENERIC;
     o.gpu.colOneRegion = flags & OWL_COL_ONE_REGION;
+    o.gpu.colOnePair = flags & OWL_COL_ONE_PAIR;
     *out = new owl_server_s{new Server(o), {}};
   });
 }
@@ -414,6 +418,7 @@ int owl_batch_create(const uint32_t* exponents, uint32_t n, const char* spec, in
     opt.longCarry = flags & OWL_LONG_CARRY;
     opt.generic = flags & OWL_GENERIC;
     opt.colOneRegion = flags & OWL_COL_ONE_REGION;
+    opt.colOnePair = flags & OWL_COL_ONE_PAIR;
     *out = new owl_batch_s{new GpuBatch(std::vector<u32>(exponents, exponents + n), spec ? spec : "", opt)};
   });
 }

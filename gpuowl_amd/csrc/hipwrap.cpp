@@ -23,6 +23,7 @@ extern "C" const OwlKernelEntry owl_kernel_table_row_b[];
 extern "C" const OwlKernelEntry owl_kernel_table_row_c[];
 extern "C" const OwlKernelEntry owl_kernel_table_colx_a[];
 extern "C" const OwlKernelEntry owl_kernel_table_colx_b[];
+extern "C" const OwlKernelEntry owl_kernel_table_colx_c[];
 extern "C" const OwlKernelEntry owl_kernel_table_coly_a[];
 extern "C" const OwlKernelEntry owl_kernel_table_coly_b[];
 
@@ -152,7 +153,7 @@ int owl_hip_copy(owl_queue queue, void* dst, const void* src, size_t bytes) {
 namespace {
 const std::initializer_list<const OwlKernelEntry*> kTables = {
     owl_kernel_table,       owl_kernel_table_row_a,  owl_kernel_table_row_b,  owl_kernel_table_row_c,
-    owl_kernel_table_colx_a, owl_kernel_table_colx_b, owl_kernel_table_coly_a, owl_kernel_table_coly_b};
+    owl_kernel_table_colx_a, owl_kernel_table_colx_b, owl_kernel_table_colx_c, owl_kernel_table_coly_a, owl_kernel_table_coly_b};
 }  // namespace
 
 int owl_hip_kernel_names(char* out, size_t cap, size_t* needed) {

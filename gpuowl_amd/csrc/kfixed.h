@@ -123,157 +123,215 @@ struct ColTw {
   }
   __device__ __forceinline__ T2 s(int h, int k) const { return tab[128 + h * 144 + 128 + k]; }
 };
+// Entry i of the twiddle table of the column pair (a0, a1).
+template<int H>
+__device__ __forceinline__ T2 colTwEntry(const JobDesc& J, int i, int a0, int a1) {
+  static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
+  T2 v = make_double2(0.0, 0.0);
+  if (i < 64) {
+    v = gld(J.twCol + i);
+  } else if (i < 128) {
+    if (i - 64 < H / 64) { v = gld(J.twCol + 64 * (i - 64)); }
+  } else {
+    const int h = (i - 128) / 144, r = (i - 128) % 144;
+    const int a = h ? a1 : a0;
+    if (r < 64) {
+      v = gld(J.twMidA + a * 64 + r);
+    } else if (r < 128) {
+      if (r - 64 < H / 64) { v = gld(J.twMidB + a * J.nHi + (r - 64)); }
+    } else {
+      v = gld(J.twMidS + a * 16 + (r - 128));
+    }
+  }
+  return v;
+}
 template<int H>
 __device__ __forceinline__ void fillColTw(const JobDesc& J, T2* tab, int a0, int a1, int tid, int T) {
-  static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
-  for (int i = tid; i < COLTW_SIZE; i += T) {
-    T2 v = make_double2(0.0, 0.0);
-    if (i < 64) {
-      v = gld(J.twCol + i);
-    } else if (i < 128) {
-      if (i - 64 < H / 64) { v = gld(J.twCol + 64 * (i - 64)); }
-    } else {
-      const int h = (i - 128) / 144, r = (i - 128) % 144;
-      const int a = h ? a1 : a0;
-      if (r < 64) {
-        v = gld(J.twMidA + a * 64 + r);
-      } else if (r < 128) {
-        if (r - 64 < H / 64) { v = gld(J.twMidB + a * J.nHi + (r - 64)); }
-      } else {
-        v = gld(J.twMidS + a * 16 + (r - 128));
-      }
-    }
-    tab[i] = v;
-  }
+  for (int i = tid; i < COLTW_SIZE; i += T) { tab[i] = colTwEntry<H>(J, i, a0, a1); }
 }
 
 // ---------------------------------------------------------------- column pass
-// Grid (W/2, batch), T threads, LDS (2*colRegion<H>() + COLTW_SIZE)*16 bytes. Modes as k_col.
-// Twiddles come from the LDS table (ColTw).
-template<int H, int T, int C0, int C1, int C2>
+// PPG = 1: grid (W/2, batch), group q handles the column pair q; LDS: two colRegion<H>()
+// regions and one twiddle table (COLTW_SIZE entries). PPG = 2: grid (G, batch) with
+// W/4 <= G < W/2, group x handles the pairs x and x + G (when below W/2); LDS: the two
+// regions and two tables. T threads; modes as k_col.
+// With two pairs per group, the second pair's column data and twiddle-table entries
+// are requested right after the current pair's first stage and stay in registers while
+// the current pair is transformed, squared and stored, so that HBM reads overlap the
+// LDS work instead of all groups loading, computing and storing in lockstep. The next
+// table goes to the other table buffer; a barrier at the end of each pair keeps the
+// regions and that buffer free until every wave is done with them.
+template<int H, int T, int C0, int C1, int C2, int PPG = 1>
 __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, int mode, int fi, int oi) {
-  static_assert(C0 * C1 * C2 == H && C0 == 16, "plan");
+  static_assert(C0 * C1 * C2 == H && C0 == 16 && (PPG == 1 || PPG == 2), "plan");
+  constexpr bool MULTI = PPG > 1;
   extern __shared__ T2 lds[];
   constexpr int REG = colRegion<H>();
-  const ColTw tw{lds + 2 * REG};
   constexpr int NB0 = H / C0;
   constexpr int NBT = 2 * NB0;
   constexpr int MB0 = (NBT + T - 1) / T;
+  constexpr int MT = (COLTW_SIZE + T - 1) / T;
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, lgR = J.lgR;
-  const int q = blockIdx.x, tid = threadIdx.x;
-  const int a0 = q, a1 = q ? W - q : W / 2;
+  const int tid0 = threadIdx.x;
   T2* io = J.freq[fi];
-  const size_t colBase = (size_t)2 * q * R;
+  const T2* other = (mode == COL_MUL) ? J.freq[oi] : nullptr;
   const size_t blockStride = (size_t)W * R;
-  fillColTw<H>(J, lds + 2 * REG, a0, a1, tid, T);
-  __syncthreads();
+  const int per = 2 * R;
 
-  // forward stage 0 (radix C0, Ns = 1), fused with the load and the mid twiddle.
   // Lane order within 2R lanes: (column h, line-in-block r), so 2R lanes read 2R*16
   // contiguous bytes of one line block.
+  T2 xin[MB0][C0];
+  auto loadPair = [&](int q, int tid) {
+    const size_t colBase = (size_t)2 * q * R;
 #pragma unroll
-  for (int m = 0; m < MB0; ++m) {
-    const int g = tid + m * T;
-    if (NBT % T == 0 || g < NBT) {
-      const int h = (g >> lgR) & 1;
-      const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
-      const T2 base = tw.mid(h, j);
-      T2 v[C0];
+    for (int m = 0; m < MB0; ++m) {
+      const int g = tid + m * T;
+      if (NBT % T == 0 || g < NBT) {
+        const int h = (g >> lgR) & 1;
+        const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
 #pragma unroll
-      for (int k = 0; k < C0; ++k) {
-        const int l = j + k * NB0;
-        T2 x = gld(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)));
-        v[k] = cmul(x, k ? cmul(base, tw.s(h, k)) : base);
-      }
-      Dft<C0, false>::run(v);
-#pragma unroll
-      for (int k = 0; k < C0; ++k) { lds[h * REG + padi(j * C0 + k)] = v[k]; }
-    }
-  }
-  __syncthreads();
-  XStages<H, REG, 2, T, false, false, C0, C1, C2>::run(lds, tw, tid);
-
-  const int per = 2 * R;
-  if (mode == COL_FWD) {
-    for (int e = tid; e < 2 * H; e += T) {
-      int lb = e / per;
-      int wi = e - lb * per;
-      int h = wi >> lgR;
-      int b = lb * R + (wi & (R - 1));
-      gst(io + (size_t)lb * blockStride + colBase + wi, lds[h * REG + padi(b)]);
-    }
-    return;
-  }
-
-  const T2* other = (mode == COL_MUL) ? J.freq[oi] : nullptr;
-  auto spos = [&](int h, int b) -> size_t { return (size_t)(b >> lgR) * blockStride + colBase + (h << lgR) + (b & (R - 1)); };
-  if (q) {
-    const T2 wa = gld(J.twPair + a0);
-    for (int b = tid; b < H; b += T) {
-      int bp = H - 1 - b;
-      T2 X = lds[padi(b)], Xp = lds[REG + padi(bp)];
-      T2 w = cmul(wa, tw(b));
-      if (other) {
-        pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(1, bp)), w);
-      } else {
-        pairSq(X, Xp, w);
-      }
-      lds[padi(b)] = X;
-      lds[REG + padi(bp)] = Xp;
-    }
-  } else {
-    const T2 wa = gld(J.twPair + W / 2);
-    for (int i = tid; i <= H; i += T) {
-      if (i <= H / 2) {
-        int b = i, bp = (H - i) % H;
-        T2 X = lds[padi(b)], Xp = lds[padi(bp)];
-        T2 w = tw(b);
-        if (other) {
-          pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(0, bp)), w);
-        } else {
-          pairSq(X, Xp, w);
+        for (int k = 0; k < C0; ++k) {
+          const int l = j + k * NB0;
+          xin[m][k] = gld(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)));
         }
-        lds[padi(b)] = X;
-        if (bp != b) { lds[padi(bp)] = Xp; }
-      } else {
-        int b = i - H / 2 - 1, bp = H - 1 - b;
-        T2 X = lds[REG + padi(b)], Xp = lds[REG + padi(bp)];
-        T2 w = cmul(wa, tw(b));
-        if (other) {
-          pairMul(X, Xp, gld(other + spos(1, b)), gld(other + spos(1, bp)), w);
-        } else {
-          pairSq(X, Xp, w);
-        }
-        lds[REG + padi(b)] = X;
-        lds[REG + padi(bp)] = Xp;
       }
     }
-  }
-  __syncthreads();
-  XStages<H, REG, 2, T, true, false, 1, C2, C1>::run(lds, tw, tid);
+  };
 
-  // last inverse stage (radix C0, Ns = NB0), fused with the conjugate mid twiddle and the store
+  int q = blockIdx.x;
+  // the column data is requested before the twiddle table is filled, so the two latencies overlap
+  loadPair(q, tid0);
+  fillColTw<H>(J, lds + 2 * REG, q, q ? W - q : W / 2, tid0, T);
+  __syncthreads();
+  for (int it = 0;; ++it) {
+    // a fresh copy of the thread index per pair: offsets derived from it are recomputed
+    // in each pair instead of being hoisted out of the loop and held in registers
+    int tid = tid0;
+    if constexpr (MULTI) { asm volatile("" : "+v"(tid)); }
+    const ColTw tw{lds + 2 * REG + (it & 1) * COLTW_SIZE};
+    const int a0 = q, a1 = q ? W - q : W / 2;
+    const size_t colBase = (size_t)2 * q * R;
+    const T2 wa = gld(J.twPair + (q ? a0 : a1));  // column a0's pairing twiddle, or column W/2's for q = 0
+    const int qn = MULTI ? q + (int)gridDim.x : W / 2;
+
+    // forward stage 0 (radix C0, Ns = 1), fused with the load and the mid twiddle
 #pragma unroll
-  for (int m = 0; m < MB0; ++m) {
-    const int g = tid + m * T;
-    if (NBT % T == 0 || g < NBT) {
-      const int h = (g >> lgR) & 1;
-      const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
-      T2 v[C0];
+    for (int m = 0; m < MB0; ++m) {
+      const int g = tid + m * T;
+      if (NBT % T == 0 || g < NBT) {
+        const int h = (g >> lgR) & 1;
+        const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
+        const T2 base = tw.mid(h, j);
+        T2 v[C0];
 #pragma unroll
-      for (int k = 0; k < C0; ++k) { v[k] = lds[h * REG + padi(j + k * NB0)]; }
+        for (int k = 0; k < C0; ++k) { v[k] = cmul(xin[m][k], k ? cmul(base, tw.s(h, k)) : base); }
+        Dft<C0, false>::run(v);
 #pragma unroll
-      for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw(j * k)); }
-      Dft<C0, true>::run(v);
-      const T2 base = tw.mid(h, j);
-#pragma unroll
-      for (int k = 0; k < C0; ++k) {
-        const int l = j + k * NB0;
-        gst(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)),
-            cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base));
+        for (int k = 0; k < C0; ++k) { lds[h * REG + padi(j * C0 + k)] = v[k]; }
       }
     }
+    // prefetch of the next pair: table entries first (written to LDS below, so their wait
+    // does not cover the column data), then the column data
+    T2 twn[MT];
+    if (MULTI && qn < W / 2) {
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int i = tid + m * T;
+        if (COLTW_SIZE % T == 0 || i < COLTW_SIZE) { twn[m] = colTwEntry<H>(J, i, qn, W - qn); }
+      }
+      loadPair(qn, tid);
+    }
+    __syncthreads();
+    XStages<H, REG, 2, T, false, false, C0, C1, C2>::run(lds, tw, tid);
+    if (MULTI && qn < W / 2) {
+      T2* tabn = lds + 2 * REG + ((it + 1) & 1) * COLTW_SIZE;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int i = tid + m * T;
+        if (COLTW_SIZE % T == 0 || i < COLTW_SIZE) { tabn[i] = twn[m]; }
+      }
+    }
+
+    if (mode == COL_FWD) {
+      for (int e = tid; e < 2 * H; e += T) {
+        int lb = e / per;
+        int wi = e - lb * per;
+        int h = wi >> lgR;
+        int b = lb * R + (wi & (R - 1));
+        gst(io + (size_t)lb * blockStride + colBase + wi, lds[h * REG + padi(b)]);
+      }
+    } else {
+      auto spos = [&](int h, int b) -> size_t { return (size_t)(b >> lgR) * blockStride + colBase + (h << lgR) + (b & (R - 1)); };
+      if (q) {
+        for (int b = tid; b < H; b += T) {
+          int bp = H - 1 - b;
+          T2 X = lds[padi(b)], Xp = lds[REG + padi(bp)];
+          T2 w = cmul(wa, tw(b));
+          if (other) {
+            pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(1, bp)), w);
+          } else {
+            pairSq(X, Xp, w);
+          }
+          lds[padi(b)] = X;
+          lds[REG + padi(bp)] = Xp;
+        }
+      } else {
+        for (int i = tid; i <= H; i += T) {
+          if (i <= H / 2) {
+            int b = i, bp = (H - i) % H;
+            T2 X = lds[padi(b)], Xp = lds[padi(bp)];
+            T2 w = tw(b);
+            if (other) {
+              pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(0, bp)), w);
+            } else {
+              pairSq(X, Xp, w);
+            }
+            lds[padi(b)] = X;
+            if (bp != b) { lds[padi(bp)] = Xp; }
+          } else {
+            int b = i - H / 2 - 1, bp = H - 1 - b;
+            T2 X = lds[REG + padi(b)], Xp = lds[REG + padi(bp)];
+            T2 w = cmul(wa, tw(b));
+            if (other) {
+              pairMul(X, Xp, gld(other + spos(1, b)), gld(other + spos(1, bp)), w);
+            } else {
+              pairSq(X, Xp, w);
+            }
+            lds[REG + padi(b)] = X;
+            lds[REG + padi(bp)] = Xp;
+          }
+        }
+      }
+      __syncthreads();
+      XStages<H, REG, 2, T, true, false, 1, C2, C1>::run(lds, tw, tid);
+
+      // last inverse stage (radix C0, Ns = NB0), fused with the conjugate mid twiddle and the store
+#pragma unroll
+      for (int m = 0; m < MB0; ++m) {
+        const int g = tid + m * T;
+        if (NBT % T == 0 || g < NBT) {
+          const int h = (g >> lgR) & 1;
+          const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
+          T2 v[C0];
+#pragma unroll
+          for (int k = 0; k < C0; ++k) { v[k] = lds[h * REG + padi(j + k * NB0)]; }
+#pragma unroll
+          for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw(j * k)); }
+          Dft<C0, true>::run(v);
+          const T2 base = tw.mid(h, j);
+#pragma unroll
+          for (int k = 0; k < C0; ++k) {
+            const int l = j + k * NB0;
+            gst(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)),
+                cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base));
+          }
+        }
+      }
+    }
+    if (!MULTI || qn >= W / 2) { break; }
+    __syncthreads();
+    q = qn;
   }
 }
 

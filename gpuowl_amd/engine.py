@@ -40,12 +40,14 @@ def int_to_words(x, E):
 
 class Gpu:
     """One exponent on the GPU: Gpu(E, spec="", long_carry=False, line_block=0, generic=False,
-    col_one_region=False)."""
+    col_one_region=False, col_one_pair=False)."""
 
-    def __init__(self, E, spec="", long_carry=False, line_block=0, generic=False, col_one_region=False):
+    def __init__(self, E, spec="", long_carry=False, line_block=0, generic=False, col_one_region=False,
+                 col_one_pair=False):
         self.lib = load()
         self.h = C.c_void_p()
-        flags = (1 if long_carry else 0) | (2 if generic else 0) | (4 if col_one_region else 0)
+        flags = (1 if long_carry else 0) | (2 if generic else 0) | (4 if col_one_region else 0) | \
+            (8 if col_one_pair else 0)
         check(self.lib.owl_gpu_create(E, (spec or "").encode(), flags, line_block, C.byref(self.h)))
         info = GpuInfo()
         check(self.lib.owl_gpu_info(self.h, C.byref(info)))

@@ -50,8 +50,10 @@ int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen,
  * flags: OWL_LONG_CARRY = full carry every iteration (the reference's -carry long);
  *        OWL_GENERIC = runtime-plan kernels even where a shape-specialized pass exists;
  *        OWL_COL_ONE_REGION = the column pass variant that takes the two columns through one
- *        LDS region in turn (pairing in registers) where the two-region pass exists. */
-enum { OWL_LONG_CARRY = 1, OWL_GENERIC = 2, OWL_COL_ONE_REGION = 4 };
+ *        LDS region in turn (pairing in registers) where the two-region pass exists;
+ *        OWL_COL_ONE_PAIR = the two-region column pass with one column pair per group where
+ *        the variant with two pairs per group would be chosen. */
+enum { OWL_LONG_CARRY = 1, OWL_GENERIC = 2, OWL_COL_ONE_REGION = 4, OWL_COL_ONE_PAIR = 8 };
 int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, owl_gpu* gpu);
 int owl_gpu_destroy(owl_gpu gpu);
 int owl_gpu_info(owl_gpu gpu, owl_gpu_info_t* info);
@@ -164,7 +166,7 @@ int owl_server_now(owl_server server, double* secs);
  * Tune file lines: "N width lineBlock flags us". */
 typedef struct {
   uint32_t width, lineBlock;
-  int flags;          /* OWL_GENERIC | OWL_COL_ONE_REGION */
+  int flags;          /* OWL_GENERIC | OWL_COL_ONE_REGION | OWL_COL_ONE_PAIR */
   double usPerIter;
   uint32_t W, H, R;
   int longCarry;

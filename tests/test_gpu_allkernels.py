@@ -35,9 +35,10 @@ def config_for(name):
         W, R = int(m.group(1)), int(m.group(2))
         H = {2048: 256, 4096: 128}.get(W, 512)
         return reference_spec(W * H), W, R, 0, name, None
-    kind, H = re.match(r"(col[xy])(\d+)", name).groups()
+    kind, H, pairs = re.match(r"(col[xy])(\d+)(p2)?$", name).groups()
     H = int(H)
-    for W in (256, 1024, 512):
+    # the two-pair column pass is chosen where W/4 groups give every CU one: W = 1024 on MI355X
+    for W in ((1024,) if pairs else (256, 1024, 512)):
         spec = reference_spec(W * H)
         if spec:
             return spec, W, 2, 4 if kind == "coly" else 0, None, name

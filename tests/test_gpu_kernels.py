@@ -12,14 +12,15 @@ from tests.fftmodel import Model
 
 pytestmark = pytest.mark.gpu
 
-# (spec, bits per word, force the runtime-plan kernels)
 # (spec, bits per word, kernel variant): "default" = shape-specialized passes where compiled
-# (two-region column pass), "coly" = the one-region column pass, "generic" = runtime-plan kernels
+# (two-region column pass, two column pairs per group at 1K:15:256), "coly" = the one-region
+# column pass, "onepair" = the two-region pass with one pair per group, "generic" = runtime-plan
+# kernels
 SHAPES = [("64:2:64", 12.0, "default"), ("256:2:256", 15.0, "default"), ("256:2:256", 15.0, "generic"),
           ("256:2:256", 15.0, "coly"), ("256:3:256", 8.0, "default"), ("256:3:256", 8.0, "coly"),
           ("1K:3:256", 8.0, "default"), ("1K:3:256", 8.0, "generic"), ("1K:3:256", 8.0, "coly"),
           ("256:13:256", 8.0, "default"), ("512:9:256", 8.0, "default"), ("1K:15:256", 17.6, "default"),
-          ("1K:15:256", 17.6, "generic"), ("1K:15:256", 17.6, "coly")]
+          ("1K:15:256", 17.6, "generic"), ("1K:15:256", 17.6, "coly"), ("1K:15:256", 17.6, "onepair")]
 
 
 def rel(a, b):
@@ -29,7 +30,8 @@ def rel(a, b):
 def setup(spec, bpw, variant, seed=1):
     N = owl.best_fit(1000, spec)[1]
     E = workloads.next_prime(int(bpw * N))
-    g = owl.Gpu(E, spec, generic=variant == "generic", col_one_region=variant == "coly")
+    g = owl.Gpu(E, spec, generic=variant == "generic", col_one_region=variant == "coly",
+                col_one_pair=variant == "onepair")
     m = Model(E, g.W, g.H, g.R)
     return g, m, np.random.default_rng(seed)
 
@@ -83,9 +85,12 @@ def test_carry_out_digits(spec, bpw, variant):
 
 
 def test_kernel_selection():
-    # the 7.5M-word shape runs the shape-specialized passes; generic=True forces the runtime-plan ones
+    # the 7.5M-word shape runs the shape-specialized passes (two column pairs per group: W/4 = 256
+    # groups, one per CU); generic=True forces the runtime-plan ones
     E = workloads.next_prime(139_000_000)
     with owl.Gpu(E) as g:
+        assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840p2")
+    with owl.Gpu(E, col_one_pair=True) as g:
         assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840")
     with owl.Gpu(E, col_one_region=True) as g:
         assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "coly3840")
@@ -95,7 +100,7 @@ def test_kernel_selection():
 
 @pytest.mark.parametrize("spec,bpw,variant", [("1K:3:256", 12.0, "default"), ("1K:3:256", 12.0, "coly"),
                                               ("1K:15:256", 17.6, "default"), ("1K:15:256", 17.6, "coly"),
-                                              ("1K:15:256", 17.6, "generic")])
+                                              ("1K:15:256", 17.6, "generic"), ("1K:15:256", 17.6, "onepair")])
 def test_mul_pass(spec, bpw, variant):
     # COL_FWD on a second operand, then COL_MUL, against the model's spectrum product
     g, m, rng = setup(spec, bpw, variant, seed=4)

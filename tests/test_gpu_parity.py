@@ -60,7 +60,8 @@ def test_ll_res64_vs_gmp(spec, bpw):
         assert out.res64 == res[-1]
 
 
-VARIANTS = {"default": {}, "generic": {"generic": True}, "coly": {"col_one_region": True}}
+VARIANTS = {"default": {}, "generic": {"generic": True}, "coly": {"col_one_region": True},
+            "onepair": {"col_one_pair": True}}
 
 
 @pytest.mark.parametrize("spec,bpw,variant", [("64:2:64", 12.0, "default"), ("256:2:256", 15.0, "default"),
@@ -163,7 +164,7 @@ def test_p4000037_prp_golden(golden):
     assert out.nErrors == 0 and out.nChecks == 10 and out.maxRoe < 0.4
 
 
-@pytest.mark.parametrize("variant", ["default", "coly", "generic"])
+@pytest.mark.parametrize("variant", ["default", "coly", "generic", "onepair"])
 def test_p139000003_prp_golden_short(golden, variant):
     head, res = golden("p139000003_prp_short")
     E = head["E"]
