@@ -141,7 +141,8 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &rowT), "kernel");
-    rowLds = R * (W + W / 16 + 4) * 16 + 64;  // R regions of rowRegion<W>() (kfixed.h) + per-wave round-off
+    // R regions of rowRegion<W>() (kfixed.h) + per-wave round-off + ROWTW_SIZE-entry twiddle table (W >= 512)
+    rowLds = R * (W + W / 16 + 4) * 16 + 64 + (W >= 512 ? 128 * 16 : 0);
   } else {
     rowKernel = "row" + std::to_string(rowT);
   }

@@ -19,6 +19,8 @@
 // lanes of a b128 access span several regions, spread over the banks.
 #pragma once
 
+#include <type_traits>
+
 #include "kcommon.h"
 
 __device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
@@ -108,6 +110,23 @@ struct GlobalTw {
   const T2* t;
   __device__ __forceinline__ T2 operator()(int m) const { return gld(t + m); }
 };
+
+// Row-pass twiddles staged in LDS (ROWTW_SIZE entries after the line regions and the
+// round-off slots): w_W^m = TA[m & 63] * TB[m >> 6], TA[i] = w_W^i, TB[j] = w_W^(64 j),
+// each twiddle a product of two correctly rounded table entries. Used for W >= 512; at
+// W = 256 the fill (68 entries for a 16 KB block) costs more than the global reads save.
+constexpr int ROWTW_SIZE = 128;
+struct RowTw {
+  const T2* tab;
+  __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[m & 63], tab[64 + (m >> 6)]); }
+};
+template<int W, int T>
+__device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
+  static_assert(W % 64 == 0 && W / 64 <= 64, "twiddle table");
+  constexpr int N = 64 + W / 64;
+#pragma unroll
+  for (int i = tid; i < N; i += T) { tab[i] = gld(twRow + (i < 64 ? i : 64 * (i - 64))); }
+}
 
 // Column-pass twiddles staged in LDS after the data regions (COLTW_SIZE entries):
 // w_H^m = TA[m & 63] * TB[m >> 6] for m < H, and for column h of the pair the mid
@@ -477,7 +496,8 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
 }
 
 // ---------------------------------------------------------------- row pass
-// Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 bytes.
+// Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 + 64
+// (+ ROWTW_SIZE*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle table.
 // Modes, options and the tagged stairway as k_row.
 template<int W, int R, int F0, int F1, int F2>
 __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts,
@@ -488,7 +508,11 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   constexpr int REG = rowRegion<W>();
   extern __shared__ T2 lds[];
   float* roePart = reinterpret_cast<float*>(lds + R * REG);  // one float per wave, after the line regions
+  T2* twTab = lds + R * REG + 4;                               // after the 64 bytes of round-off slots
   const JobDesc& J = jobs[blockIdx.y];
+  constexpr bool LDS_TW = W >= 512;
+  using Tw = std::conditional_t<LDS_TW, RowTw, GlobalTw>;
+  const Tw tw{LDS_TW ? twTab : J.twRow};
   const int nb = J.nb;
   __shared__ int ticketSlot;
   const int g = (mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x;
@@ -496,6 +520,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   const int tid = threadIdx.x;
   const bool ll = opts & 1, mul3 = opts & 2;
   T2* blk = J.freq[fi] + (size_t)lb * LW;
+  if constexpr (LDS_TW) { fillRowTw<W, T>(J.twRow, twTab, tid); }  // read after the first barrier below
 
   // Thread tid owns columns c = tid + k*T (k < F0) of the R lines of the block.
   T2 v[R][F0];
@@ -531,7 +556,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       lds[r * REG + padi(slotFreq(s, W))] = gld(blk + e);
     }
     __syncthreads();
-    XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, GlobalTw{J.twRow}, tid);
+    XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, tw, tid);
     __syncthreads();
     // last inverse stage (radix F0, Ns = T): butterfly tid of each line -> columns tid + k*T
 #pragma unroll
@@ -539,7 +564,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
 #pragma unroll
       for (int k = 0; k < F0; ++k) { v[r][k] = lds[r * REG + padi(tid + k * T)]; }
 #pragma unroll
-      for (int k = 1; k < F0; ++k) { v[r][k] = cmulc(v[r][k], gld(J.twRow + tid * k)); }
+      for (int k = 1; k < F0; ++k) { v[r][k] = cmulc(v[r][k], tw(tid * k)); }
       Dft<F0, true>::run(v[r]);
     }
 
@@ -657,7 +682,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
     for (int k = 0; k < F0; ++k) { lds[r * REG + padi(tid * F0 + k)] = v[r][k]; }
   }
   __syncthreads();
-  XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, GlobalTw{J.twRow}, tid);
+  XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, tw, tid);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < LW / T; ++i) {
