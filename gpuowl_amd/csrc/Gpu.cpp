@@ -136,7 +136,6 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   }
   rowLds = R * W * 16;
   colLds = 2 * H * 16;
-  colGroups = W / 2;
   if (rowFixed) {
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
@@ -147,24 +146,18 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     rowKernel = "row" + std::to_string(rowT);
   }
   if (colFixed) {
-    u32 tables = 1;
-    // Two column pairs per group ("colx<H>p2", compiled where one k_colx group fills a CU's
-    // LDS) when W/4 groups still give every CU one: the second pair's loads then overlap the
-    // first pair's work. One more twiddle table in LDS.
-    if (useX && !opt.colOnePair && haveKernel(colX + "p2")) {
-      int cus = 0;
-      owlCheck(owl_hip_device_info(opt.device, nullptr, 0, nullptr, nullptr, &cus, nullptr), "device info");
-      if (W % 4 == 0 && cus > 0 && W / 4 >= u32(cus)) {
-        colKernel = colX + "p2";
-        colGroups = W / 4;
-        tables = 2;
-      }
-    }
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
-    // colRegion<H>() regions + COLTW_SIZE-entry tables (kfixed.h)
-    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + 416 * tables) * 16;
+    // colRegion<H>() regions + a COLTW_SIZE-entry twiddle table (kfixed.h)
+    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + 416) * 16;
+    if (useX && !opt.colOnePair && haveKernel(colX + "m")) {
+      int cus = 0, lds = 0;
+      owlCheck(owl_hip_device_info(opt.device, nullptr, 0, nullptr, nullptr, &cus, &lds), "device info");
+      colMulti = true;
+      colLdsM = colLds + 416 * 16;  // the next pair's twiddle table
+      slotsM = u32(std::max(cus, 1)) * std::max<u32>(1, u32(lds) / colLdsM);
+    }
   } else {
     colKernel = "col" + std::to_string(colT);
   }
@@ -238,7 +231,10 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   jobs_.write(queue_, {J});
 
   kRow = Kernel(rowKernel);
-  kCol = Kernel(colKernel);
+  colOne_ = colKernel;
+  kCol = Kernel(colOne_);
+  if (colMulti) { kColM = Kernel(colOne_ + "m"); }
+  colKernel = colKernelFor(1);
   kCarryB = Kernel("carryB");
   kCarryFix = Kernel("carryFix");
   kWordsIn = Kernel("wordsIn");
@@ -362,20 +358,36 @@ void Gpu::carryAll(Buf b, Launch L) {
   kCarryFix(L.q, 1, L.n, 64, 0, {&L.jobs, &wi, &co});
 }
 
-void Gpu::colSquare(Launch L) {
-  int mode = COL_SQUARE, fi = 0, oi = 0;
-  kCol(L.q, colGroups, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
+// Pairs per column group for a launch of nJobs jobs: the largest power of two up to
+// kMaxPairs whose grid (W/2 / pairs groups per job) still gives each of the device's
+// slotsM group slots a group; 1 when the multi-pair pass is not available. (At 7.5M words
+// two pairs per group take the column pass from 60.2 to 55.3 us; in 64-job batches of
+// 1.5M words, 2, 4, 8 or 16 pairs were within 2% of one pair and not kept for H < 2304.)
+u32 Gpu::colPairsPerGroup(u32 nJobs) const {
+  constexpr u32 kMaxPairs = 2;
+  u32 best = 1;
+  if (!colMulti) { return best; }
+  for (u32 p = 2; p <= kMaxPairs && p <= W / 2; p *= 2) {
+    if (u64(W / 2 / p) * nJobs < slotsM) { break; }
+    best = p;
+  }
+  return best;
 }
 
-void Gpu::colMul(Launch L) {
-  int mode = COL_MUL, fi = 0, oi = 2;
-  kCol(L.q, colGroups, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
+std::string Gpu::colKernelFor(u32 nJobs) const { return colPairsPerGroup(nJobs) > 1 ? colOne_ + "m" : colOne_; }
+
+void Gpu::colPass(int mode, int fi, int oi, Launch L) {
+  const u32 p = colPairsPerGroup(L.n);
+  if (p > 1) {
+    kColM(L.q, W / 2 / p, L.n, colT, colLdsM, {&L.jobs, &mode, &fi, &oi});
+  } else {
+    kCol(L.q, W / 2, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
+  }
 }
 
-void Gpu::colFwdSpec(Launch L) {
-  int mode = COL_FWD, fi = 2, oi = 2;
-  kCol(L.q, colGroups, L.n, colT, colLds, {&L.jobs, &mode, &fi, &oi});
-}
+void Gpu::colSquare(Launch L) { colPass(COL_SQUARE, 0, 0, L); }
+void Gpu::colMul(Launch L) { colPass(COL_MUL, 0, 2, L); }
+void Gpu::colFwdSpec(Launch L) { colPass(COL_FWD, 2, 2, L); }
 
 // ---------------------------------------------------------------- squaring
 void Gpu::square(Buf out, Buf in, bool leadIn, bool leadOut, bool doMul3, bool doLL) {

@@ -75,8 +75,15 @@ public:
   const u32 E;
   const FFTConfig fft;
   u32 N, ND, W, H, R, nb, rowT, colT, rowLds, colLds;
-  u32 colGroups;  // column-pass groups per job (W/2 = one pair each)
-  std::string rowKernel, colKernel;
+  std::string rowKernel, colKernel;  // colKernel: what a launch of this job alone runs
+  // Column pass with several column pairs per group ("colx<H>m", compiled for every k_colx
+  // shape): colLdsM bytes of LDS (one more twiddle table), slotsM = groups the device holds
+  // at once. colPairsPerGroup(n) is 1 (the one-pair "colx<H>") unless a launch of n jobs
+  // still gives every slot a group with fewer groups per job.
+  bool colMulti = false;
+  u32 colLdsM = 0, slotsM = 0;
+  u32 colPairsPerGroup(u32 nJobs) const;
+  std::string colKernelFor(u32 nJobs) const;  // the column kernel a launch of nJobs jobs runs
   bool useLongCarry;
   std::vector<u32> rowPlan, colPlan;
 
@@ -144,6 +151,7 @@ public:
   void colSquare(Launch L);
   void colMul(Launch L);
   void colFwdSpec(Launch L);
+  void colPass(int mode, int fi, int oi, Launch L);
   void carryAll(Buf b, Launch L);
   void square(Buf out, Buf in, bool leadIn, bool leadOut, bool doMul3, bool doLL, Launch L);
   u32 squareLoop(Buf out, Buf in, u32 from, u32 to, bool doTailMul3, bool doLL, Launch L);
@@ -191,7 +199,8 @@ private:
   u32 stairTag_ = 0, lastTag_ = 0, resLo_ = 1, resHi_ = 0;
   u32 faultAt_ = 0;
   bool tunedLineBlock_ = false;
-  Kernel kRow, kCol, kCarryB, kCarryFix, kWordsIn, kWordsOut, kResidue, kIsEqual, kSum64, kCopy;
+  std::string colOne_;  // the one-pair column kernel
+  Kernel kRow, kCol, kColM, kCarryB, kCarryFix, kWordsIn, kWordsOut, kResidue, kIsEqual, kSum64, kCopy;
 
   template<typename T> T* alloc(size_t n);
   template<typename T> T* upload(const std::vector<T>& v);

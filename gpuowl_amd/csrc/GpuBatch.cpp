@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <map>
+#include <stdexcept>
 
 namespace {
 double now() {
@@ -32,6 +33,15 @@ GpuBatch::GpuBatch(const std::vector<u32>& exponents, const std::string& spec, c
     for (size_t i : grp.index) { d.push_back(jobs[i]->job()); }
     grp.descs = Buffer<JobDesc>(grp.lead->queue(), d);
   }
+}
+
+const GpuBatch::Group& GpuBatch::groupOf(size_t job) const {
+  for (const Group& g : groups) {
+    for (size_t i : g.index) {
+      if (i == job) { return g; }
+    }
+  }
+  throw std::invalid_argument("job index");
 }
 
 u64 GpuBatch::totalWords() const {

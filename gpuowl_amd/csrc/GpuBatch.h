@@ -25,6 +25,7 @@ public:
   std::vector<Group> groups;
 
   size_t size() const { return jobs.size(); }
+  const Group& groupOf(size_t job) const;
   u64 totalWords() const;
   void finish();
 

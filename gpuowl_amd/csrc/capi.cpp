@@ -455,6 +455,9 @@ int owl_batch_job_info(owl_batch b, uint32_t job, owl_gpu_info_t* info) {
   return guarded([&] {
     if (job >= b->batch->size()) { throw std::invalid_argument("job index"); }
     fillInfo(*b->batch->jobs[job], info);
+    // the column kernel of the job's group launches (several pairs per group in a large group)
+    const GpuBatch::Group& g = b->batch->groupOf(job);
+    snprintf(info->colKernel, sizeof(info->colKernel), "%s", g.lead->colKernelFor(u32(g.index.size())).c_str());
   });
 }
 

@@ -170,20 +170,19 @@ __device__ __forceinline__ void fillColTw(const JobDesc& J, T2* tab, int a0, int
 }
 
 // ---------------------------------------------------------------- column pass
-// PPG = 1: grid (W/2, batch), group q handles the column pair q; LDS: two colRegion<H>()
-// regions and one twiddle table (COLTW_SIZE entries). PPG = 2: grid (G, batch) with
-// W/4 <= G < W/2, group x handles the pairs x and x + G (when below W/2); LDS: the two
-// regions and two tables. T threads; modes as k_col.
-// With two pairs per group, the second pair's column data and twiddle-table entries
+// MULTI = false: grid (W/2, batch), group q handles the column pair q; LDS: two
+// colRegion<H>() regions and one twiddle table (COLTW_SIZE entries).
+// MULTI = true: grid (G, batch) with G <= W/2, group x handles the pairs x, x + G, x + 2G,
+// ... below W/2; LDS: the two regions and two tables. T threads; modes as k_col.
+// With several pairs per group, the next pair's column data and twiddle-table entries
 // are requested right after the current pair's first stage and stay in registers while
 // the current pair is transformed, squared and stored, so that HBM reads overlap the
-// LDS work instead of all groups loading, computing and storing in lockstep. The next
+// LDS work instead of each group waiting for its loads with nothing else to do. The next
 // table goes to the other table buffer; a barrier at the end of each pair keeps the
 // regions and that buffer free until every wave is done with them.
-template<int H, int T, int C0, int C1, int C2, int PPG = 1>
+template<int H, int T, int C0, int C1, int C2, bool MULTI = false>
 __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, int mode, int fi, int oi) {
-  static_assert(C0 * C1 * C2 == H && C0 == 16 && (PPG == 1 || PPG == 2), "plan");
-  constexpr bool MULTI = PPG > 1;
+  static_assert(C0 * C1 * C2 == H && C0 == 16, "plan");
   extern __shared__ T2 lds[];
   constexpr int REG = colRegion<H>();
   constexpr int NB0 = H / C0;

@@ -1,5 +1,6 @@
 """Every compiled shape-specialized pass, forced through the tune table onto a geometry
-it serves, gives GMP's PRP residues (with Gerbicz checks) over a short run."""
+it serves, gives GMP's PRP residues (with Gerbicz checks) over a short run; the multi-pair
+column passes run in a batch large enough to choose them."""
 import re
 import sys
 
@@ -35,10 +36,11 @@ def config_for(name):
         W, R = int(m.group(1)), int(m.group(2))
         H = {2048: 256, 4096: 128}.get(W, 512)
         return reference_spec(W * H), W, R, 0, name, None
-    kind, H, pairs = re.match(r"(col[xy])(\d+)(p2)?$", name).groups()
+    kind, H, multi = re.match(r"(col[xy])(\d+)(m)?$", name).groups()
     H = int(H)
-    # the two-pair column pass is chosen where W/4 groups give every CU one: W = 1024 on MI355X
-    for W in ((1024,) if pairs else (256, 1024, 512)):
+    if multi:  # W = 1024 has the most column pairs per job to share out
+        return reference_spec(1024 * H), 1024, 2, 0, None, name
+    for W in (256, 1024, 512):
         spec = reference_spec(W * H)
         if spec:
             return spec, W, 2, 4 if kind == "coly" else 0, None, name
@@ -50,6 +52,25 @@ def test_specialized_kernel_is_exact(name):
     cfg = config_for(name)
     assert cfg and cfg[0], f"no reference shape serves {name}"
     spec, W, R, flags, row, col = cfg
+    if name.endswith("m"):
+        # the multi-pair column pass (compiled for H >= 2304, one group per CU) runs where a
+        # launch has groups to spare; a batch of 2 jobs gives 2 * W/4 = 512 groups of two pairs
+        N = owl.best_fit(1000, spec)[1]
+        exps = [workloads.next_prime(15 * N)]
+        while len(exps) < 2:
+            exps.append(workloads.next_prime(exps[-1] + 1))
+        announce(name, spec, W, R, exps[0], "x", len(exps))
+        owl.tune_set(N, W, R, flags)
+        try:
+            with owl.Batch(exps, spec) as b:
+                assert b.job_info(0)["col"] == name
+                outs, _ = b.prp(60, block_size=30, check_step=30)
+            for E, o in zip(exps, outs):
+                assert o.res64 == oracle.gmp_res64(E, 60, 60)[0], E
+                assert o.nErrors == 0 and o.maxRoe < 0.4
+        finally:
+            owl.tune_clear()
+        return
     N = owl.best_fit(1000, spec)[1]
     E = workloads.next_prime(15 * N)
     announce(name, spec, W, R, flags, E)

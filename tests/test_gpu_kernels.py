@@ -89,7 +89,7 @@ def test_kernel_selection():
     # groups, one per CU); generic=True forces the runtime-plan ones
     E = workloads.next_prime(139_000_000)
     with owl.Gpu(E) as g:
-        assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840p2")
+        assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840m")
     with owl.Gpu(E, col_one_pair=True) as g:
         assert (g.row_kernel, g.col_kernel) == ("rowx1024r4", "colx3840")
     with owl.Gpu(E, col_one_region=True) as g:

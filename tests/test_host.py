@@ -119,5 +119,5 @@ def test_tune_table_round_trip(tmp_path):
 def test_kernel_names_cover_the_specialized_passes():
     names = gpuowl_amd.kernel_names()
     assert len(names) == len(set(names))
-    for n in ("rowx1024r4", "rowx256r2", "colx3840", "colx3840p2", "coly3840", "coly512", "carryB", "readResidue"):
+    for n in ("rowx1024r4", "rowx256r2", "colx3840", "colx3840m", "coly3840", "coly512", "carryB", "readResidue"):
         assert n in names
