@@ -25,6 +25,24 @@ __device__ __forceinline__ void gst(T2* p, T2 x) {
   v2d v = {x.x, x.y};
   *(GLOBAL v2d*)p = v;
 }
+// Streaming accesses for data read or written once per pass (non-temporal: the line
+// is not kept in the caches for reuse); NT = false gives the plain access.
+template<bool NT> __device__ __forceinline__ T2 ldx(const T2* p) {
+  if constexpr (NT) {
+    v2d v = __builtin_nontemporal_load((const GLOBAL v2d*)p);
+    return make_double2(v.x, v.y);
+  } else {
+    return gld(p);
+  }
+}
+template<bool NT> __device__ __forceinline__ void stx(T2* p, T2 x) {
+  if constexpr (NT) {
+    v2d v = {x.x, x.y};
+    __builtin_nontemporal_store(v, (GLOBAL v2d*)p);
+  } else {
+    gst(p, x);
+  }
+}
 __device__ __forceinline__ int2 gld(const int2* p) {
   v2i v = *(const GLOBAL v2i*)p;
   return make_int2(v.x, v.y);

@@ -192,6 +192,9 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, lgR = J.lgR;
   const int tid0 = threadIdx.x;
+  // the column data streams through with non-temporal accesses from H = 768 up (at 512 x
+  // 256 the 2 MB buffers stay cached between passes and plain accesses measured faster)
+  constexpr bool NT = H >= 768;
   T2* io = J.freq[fi];
   const T2* other = (mode == COL_MUL) ? J.freq[oi] : nullptr;
   const size_t blockStride = (size_t)W * R;
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
 #pragma unroll
         for (int k = 0; k < C0; ++k) {
           const int l = j + k * NB0;
-          xin[m][k] = gld(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)));
+          xin[m][k] = ldx<NT>(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)));
         }
       }
     }
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
         int wi = e - lb * per;
         int h = wi >> lgR;
         int b = lb * R + (wi & (R - 1));
-        gst(io + (size_t)lb * blockStride + colBase + wi, lds[h * REG + padi(b)]);
+        stx<NT>(io + (size_t)lb * blockStride + colBase + wi, lds[h * REG + padi(b)]);
       }
     } else {
       auto spos = [&](int h, int b) -> size_t { return (size_t)(b >> lgR) * blockStride + colBase + (h << lgR) + (b & (R - 1)); };
@@ -341,8 +344,8 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
 #pragma unroll
           for (int k = 0; k < C0; ++k) {
             const int l = j + k * NB0;
-            gst(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)),
-                cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base));
+            stx<NT>(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)),
+                    cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base));
           }
         }
       }
@@ -518,6 +521,8 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   const int lb = (g == nb) ? 0 : g;
   const int tid = threadIdx.x;
   const bool ll = opts & 1, mul3 = opts & 2;
+  // the block streams through with non-temporal accesses from W = 512 up
+  constexpr bool NT = W >= 512;
   T2* blk = J.freq[fi] + (size_t)lb * LW;
   if constexpr (LDS_TW) { fillRowTw<W, T>(J.twRow, twTab, tid); }  // read after the first barrier below
 
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
     for (int i = 0; i < LW / T; ++i) {
       const int e = tid + i * T;
       const int s = e / R, r = e % R;
-      lds[r * REG + padi(slotFreq(s, W))] = gld(blk + e);
+      lds[r * REG + padi(slotFreq(s, W))] = ldx<NT>(blk + e);
     }
     __syncthreads();
     XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, tw, tid);
@@ -687,7 +692,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   for (int i = 0; i < LW / T; ++i) {
     const int e = tid + i * T;
     const int s = e / R, r = e % R;
-    gst(blk + e, lds[r * REG + padi(slotFreq(s, W))]);
+    stx<NT>(blk + e, lds[r * REG + padi(slotFreq(s, W))]);
   }
 }
 
