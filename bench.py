@@ -217,6 +217,10 @@ def main():
     import gpuowl_amd as owl
     from gpuowl_amd import workloads
 
+    def note(msg):
+        # progress on stderr as each leg ends; stdout carries only the JSON line
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
+
     if a.profile_batch:
         b = owl.Batch(workloads.uniform64_20m(), "1K:3:256")
         us = b.time_squares(a.steps, a.warmup, use_graph=False)
@@ -253,6 +257,7 @@ def main():
     us_col, us_row = g.time_kernels(200)
     us_copy = g.time_copy(50)
     roe = g.read_roe()
+    note(f"{a.steps} timed iterations: {us:.2f} us/iter (row {us_row:.2f} us, column {us_col:.2f} us)")
 
     # Algorithmic bytes: 64*N per iteration, 16*N (state read once, written once) for each of
     # the reference's four passes. The row pass does one of them (carryFused), the column pass
@@ -268,6 +273,7 @@ def main():
         prefix = "k_rowx" if names[dom].startswith("rowx") else "k_colx" if names[dom].startswith("colx") else \
             "k_row" if dom == "row" else "k_col"
         traffic = pmc_traffic(prefix, copy_bytes)
+        note(f"counter passes done: {traffic['bytes'] if traffic else None} HBM bytes per launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic["bytes"] if traffic else None,
@@ -289,19 +295,23 @@ def main():
 
     batched = None
     if not a.no_batch:
-        batched = {
-            "uniform_64x20M_at_1K:3:256": leg(batch_leg, owl, workloads.uniform64_20m(), "1K:3:256", a.batch_iters, 10),
-            "uneven_200_exponents_1M_to_60M": leg(batch_leg, owl, workloads.uneven200(), "", a.batch_iters, 10),
-        }
+        batched = {}
+        batched["uniform_64x20M_at_1K:3:256"] = leg(batch_leg, owl, workloads.uniform64_20m(), "1K:3:256",
+                                                    a.batch_iters, 10)
+        note("batch of 64 exponents at 1K:3:256 done")
+        batched["uneven_200_exponents_1M_to_60M"] = leg(batch_leg, owl, workloads.uneven200(), "", a.batch_iters, 10)
+        note("batch of 200 uneven exponents done")
 
     served = None
     if not a.no_serve:
         served = leg(serve_leg, owl, workloads, a.serve_requests, a.serve_lambda, a.serve_slice,
                      a.serve_full_below)
+        note("serving sample done")
 
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(E, a.cpu_iters)
+        note("CPU baseline done")
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "iter/s", "n_gpus": 1, "steps": a.steps,
             "warmup": a.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
