@@ -149,13 +149,14 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
-    // colRegion<H>() regions + a COLTW_SIZE-entry twiddle table (kfixed.h)
-    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + 416) * 16;
+    // colRegion<H>() regions + a colTwSize<H>()-entry twiddle table (kfixed.h)
+    const u32 twSize = 64 + H / 64 + 2 * (64 + H / 64 + 16);
+    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + twSize) * 16;
     if (useX && !opt.colOnePair && haveKernel(colX + "m")) {
       int cus = 0, lds = 0;
       owlCheck(owl_hip_device_info(opt.device, nullptr, 0, nullptr, nullptr, &cus, &lds), "device info");
       colMulti = true;
-      colLdsM = colLds + 416 * 16;  // the next pair's twiddle table
+      colLdsM = colLds + twSize * 16;  // the next pair's twiddle table
       slotsM = u32(std::max(cus, 1)) * std::max<u32>(1, u32(lds) / colLdsM);
     }
   } else {

@@ -128,50 +128,56 @@ __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
   for (int i = tid; i < N; i += T) { tab[i] = gld(twRow + (i < 64 ? i : 64 * (i - 64))); }
 }
 
-// Column-pass twiddles staged in LDS after the data regions (COLTW_SIZE entries):
+// Column-pass twiddles staged in LDS after the data regions (colTwSize<H>() entries):
 // w_H^m = TA[m & 63] * TB[m >> 6] for m < H, and for column h of the pair the mid
 // twiddles w_ND^(a*l) = MA[h][l & 63] * MB[h][l >> 6] and S[h][k] = w_ND^(a*k*H/16).
-// Each twiddle is a product of two correctly rounded table entries.
-constexpr int COLTW_SIZE = 64 + 64 + 2 * (64 + 64 + 16);
+// Each twiddle is a product of two correctly rounded table entries. Layout: TA[64],
+// TB[H/64], then per column MA[64], MB[H/64], S[16]; sized per H so that small-H groups
+// keep their LDS footprint (and so the number of groups per CU) as low as the data allows.
+template<int H> constexpr int colTwPer() { return 64 + H / 64 + 16; }
+template<int H> constexpr int colTwSize() { return 64 + H / 64 + 2 * colTwPer<H>(); }
+template<int H>
 struct ColTw {
+  static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
+  static constexpr int MID = 64 + H / 64;  // start of column 0's part
   const T2* tab;
   __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[m & 63], tab[64 + (m >> 6)]); }
   __device__ __forceinline__ T2 mid(int h, int l) const {
-    const T2* p = tab + 128 + h * 144;
+    const T2* p = tab + MID + h * colTwPer<H>();
     return cmul(p[l & 63], p[64 + (l >> 6)]);
   }
-  __device__ __forceinline__ T2 s(int h, int k) const { return tab[128 + h * 144 + 128 + k]; }
+  __device__ __forceinline__ T2 s(int h, int k) const { return tab[MID + h * colTwPer<H>() + 64 + H / 64 + k]; }
 };
 // Entry i of the twiddle table of the column pair (a0, a1).
 template<int H>
 __device__ __forceinline__ T2 colTwEntry(const JobDesc& J, int i, int a0, int a1) {
-  static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
-  T2 v = make_double2(0.0, 0.0);
+  constexpr int NHI = H / 64, MID = ColTw<H>::MID, PER = colTwPer<H>();
+  T2 v;
   if (i < 64) {
     v = gld(J.twCol + i);
-  } else if (i < 128) {
-    if (i - 64 < H / 64) { v = gld(J.twCol + 64 * (i - 64)); }
+  } else if (i < MID) {
+    v = gld(J.twCol + 64 * (i - 64));
   } else {
-    const int h = (i - 128) / 144, r = (i - 128) % 144;
+    const int h = (i - MID) / PER, r = (i - MID) % PER;
     const int a = h ? a1 : a0;
     if (r < 64) {
       v = gld(J.twMidA + a * 64 + r);
-    } else if (r < 128) {
-      if (r - 64 < H / 64) { v = gld(J.twMidB + a * J.nHi + (r - 64)); }
+    } else if (r < 64 + NHI) {
+      v = gld(J.twMidB + a * J.nHi + (r - 64));
     } else {
-      v = gld(J.twMidS + a * 16 + (r - 128));
+      v = gld(J.twMidS + a * 16 + (r - 64 - NHI));
     }
   }
   return v;
 }
 template<int H>
 __device__ __forceinline__ void fillColTw(const JobDesc& J, T2* tab, int a0, int a1, int tid, int T) {
-  for (int i = tid; i < COLTW_SIZE; i += T) { tab[i] = colTwEntry<H>(J, i, a0, a1); }
+  for (int i = tid; i < colTwSize<H>(); i += T) { tab[i] = colTwEntry<H>(J, i, a0, a1); }
 }
 
 // ---------------------------------------------------------------- column pass
 // MULTI = false: grid (W/2, batch), group q handles the column pair q; LDS: two
-// colRegion<H>() regions and one twiddle table (COLTW_SIZE entries).
+// colRegion<H>() regions and one twiddle table (colTwSize<H>() entries).
 // MULTI = true: grid (G, batch) with G <= W/2, group x handles the pairs x, x + G, x + 2G,
 // ... below W/2; LDS: the two regions and two tables. T threads; modes as k_col.
 // With several pairs per group, the next pair's column data and twiddle-table entries
@@ -188,7 +194,8 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
   constexpr int NB0 = H / C0;
   constexpr int NBT = 2 * NB0;
   constexpr int MB0 = (NBT + T - 1) / T;
-  constexpr int MT = (COLTW_SIZE + T - 1) / T;
+  constexpr int TWS = colTwSize<H>();
+  constexpr int MT = (TWS + T - 1) / T;
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, lgR = J.lgR;
   const int tid0 = threadIdx.x;
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
     // in each pair instead of being hoisted out of the loop and held in registers
     int tid = tid0;
     if constexpr (MULTI) { asm volatile("" : "+v"(tid)); }
-    const ColTw tw{lds + 2 * REG + (it & 1) * COLTW_SIZE};
+    const ColTw<H> tw{lds + 2 * REG + (it & 1) * TWS};
     const int a0 = q, a1 = q ? W - q : W / 2;
     const size_t colBase = (size_t)2 * q * R;
     const T2 wa = gld(J.twPair + (q ? a0 : a1));  // column a0's pairing twiddle, or column W/2's for q = 0
@@ -259,18 +266,18 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         const int i = tid + m * T;
-        if (COLTW_SIZE % T == 0 || i < COLTW_SIZE) { twn[m] = colTwEntry<H>(J, i, qn, W - qn); }
+        if (TWS % T == 0 || i < TWS) { twn[m] = colTwEntry<H>(J, i, qn, W - qn); }
       }
       loadPair(qn, tid);
     }
     __syncthreads();
     XStages<H, REG, 2, T, false, false, C0, C1, C2>::run(lds, tw, tid);
     if (MULTI && qn < W / 2) {
-      T2* tabn = lds + 2 * REG + ((it + 1) & 1) * COLTW_SIZE;
+      T2* tabn = lds + 2 * REG + ((it + 1) & 1) * TWS;
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         const int i = tid + m * T;
-        if (COLTW_SIZE % T == 0 || i < COLTW_SIZE) { tabn[i] = twn[m]; }
+        if (TWS % T == 0 || i < TWS) { tabn[i] = twn[m]; }
       }
     }
 
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
 // X1[H-1-b], is X1[(T-1-t) + T*(C2-1-k)] in the same thread. The inverse's first stage
 // (radix C2, Ns = 1) reads exactly those registers. The pair q = 0 (columns 0 and W/2,
 // each paired with itself) goes through LDS. Plan: forward [16, C1, C2], inverse
-// [C2, C1, 16]; LDS (colRegion<H>() + COLTW_SIZE)*16 bytes, twiddles from the LDS table.
+// [C2, C1, 16]; LDS (colRegion<H>() + colTwSize<H>())*16 bytes, twiddles from the LDS table.
 template<int H, int C1, int C2>
 __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ jobs, int mode, int fi, int oi) {
   constexpr int C0 = 16;
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
   static_assert(C0 * C1 * C2 == H && T % 64 == 0 && NB0 <= T, "plan");
   constexpr int REG = colRegion<H>();
   extern __shared__ T2 lds[];
-  const ColTw tw{lds + REG};
+  const ColTw<H> tw{lds + REG};
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, lgR = J.lgR;
   const int q = blockIdx.x, tid = threadIdx.x;
