@@ -95,6 +95,13 @@ SIGNATURES = {
     "owl_gpu_prp": [vp, u32, u32, u32, P(u64), u32, u32, P(PrpResult)],
     "owl_gpu_arm_fault": [vp, u32],
     "owl_gpu_ll": [vp, u32, P(u64), u32, u32, P(LLResult)],
+    "owl_gpu_prp_saved": [vp, cp, u32, u32, u32, u32, P(u64), u32, u32, P(PrpResult)],
+    "owl_gpu_ll_saved": [vp, cp, u32, u32, u32, P(u64), u32, u32, P(LLResult)],
+    "owl_savefile_write_prp": [cp, u32, u32, u32, u64, u32, dbl, P(u32), u32],
+    "owl_savefile_read_prp": [cp, P(u32), P(u32), P(u32), P(u64), P(u32), P(dbl), P(u32), u32],
+    "owl_savefile_write_ll": [cp, u32, u32, dbl, P(u32), u32],
+    "owl_savefile_read_ll": [cp, P(u32), P(u32), P(dbl), P(u32), u32],
+    "owl_crc32": [vp, C.c_size_t, P(u32)],
     "owl_gpu_time_prp": [vp, u32, u32, u32, P(dbl)],
     "owl_gpu_time_squares": [vp, u32, u32, i32, P(dbl)],
     "owl_gpu_time_kernels": [vp, u32, P(dbl), P(dbl)],

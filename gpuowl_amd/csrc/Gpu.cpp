@@ -554,13 +554,58 @@ Words makeWords(u32 E, u32 v) {
 }
 }  // namespace
 
-PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* resAt, u32 resStep) {
+void Gpu::writeState(const Words& check, u32 B) {
+  if (B < 2) { throw std::invalid_argument("block size"); }
+  // c^(2^n - 1) for n = 1, 2, 4, ... while 2n divides B with B/(2n) >= 2, then B/n - 1
+  // steps of y <- y^(2^n) * c^(2^n - 1), the last one times 3
+  writeIn(CHECK, check);
+  copy(DATA, CHECK);
+  copy(AUX, CHECK);
+  u32 n = 1;
+  for (; B % (2 * n) == 0 && B / (2 * n) >= 2; n *= 2) {
+    squareLoop(DATA, 0, n);
+    modMul(DATA, AUX);
+    copy(AUX, DATA);
+  }
+  const u32 q = B / n;
+  for (u32 i = 1; i < q; ++i) {
+    squareLoop(DATA, 0, n);
+    modMul(DATA, AUX, i == q - 1);
+  }
+}
+
+PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* resAt, u32 resStep,
+                          Saver<PRPState>* saver) {
   if (!B) { throw std::invalid_argument("block size"); }
-  if (!checkStep) { checkStep = 10 * B; }
-  if (checkStep % B) { throw std::invalid_argument("check step must be a multiple of the block size"); }
   PRPResult out;
   Words vData = makeWords(E, 3), vCheck = makeWords(E, 1);
   u32 vK = 0;
+  // a state verified in this run holds x_vK in its check already (src/Gpu.cpp:1466-1471,
+  // 1545); a savefile's check does not
+  bool vCheckHasX = false;
+  double elapsedBefore = 0;
+  if (saver) {
+    for (int attempt = 0;; ++attempt) {
+      PRPState s = saver->load();
+      if (s.k == 0) { break; }  // no savefile: the start state
+      if (s.blockSize != B && (!checkStep || checkStep % s.blockSize)) { checkStep = 0; }
+      B = s.blockSize;
+      writeState(s.check, B);
+      if (dataResidue() == s.res64) {
+        vData = readData(DATA);
+        vCheck = s.check;
+        vK = s.k;
+        out.nErrors = s.nErrors;
+        elapsedBefore = s.elapsed;
+        break;
+      }
+      if (attempt) { throw GpuError("savefile res64 does not match its check"); }
+      saver->dropMostRecent();  // try an earlier savefile (src/Gpu.cpp:1169-1191)
+    }
+  }
+  if (!checkStep) { checkStep = 10 * B; }
+  if (checkStep % B) { throw std::invalid_argument("check step must be a multiple of the block size"); }
+  const u32 nErrorsBefore = out.nErrors;
   double t0 = now();
   readROE();
   restartRoeWindow();
@@ -570,8 +615,7 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
     writeIn(CHECK, vCheck);
     u32 k = vK;
     if (resAt && resStep) { resAt->resize(vK / resStep); }
-    // a verified state already holds x_vK in the check (src/Gpu.cpp:1466-1471, 1545)
-    bool skip = vK > 0, leadIn = true, failed = false;
+    bool skip = vK > 0 && vCheckHasX, leadIn = true, failed = false;
     while (k < iters) {
       if (skip) {
         skip = false;
@@ -588,15 +632,22 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
       bool check = (k % checkStep == 0) || (k == iters && k % B == 0);
       if (check) {
         maxRoe = std::max(maxRoe, readROE().max);
+        Words saved;
+        if (saver) { saved = readData(CHECK); }  // the check before it takes x_k
         if (this->doCheck(B)) {
           ++out.nChecks;
           skip = true;
           vData = readData(DATA);
           vCheck = readData(CHECK);
+          vCheckHasX = true;
           vK = k;
+          if (saver && k < E) {
+            saved.resize(nWords(E), 0);
+            saver->save({E, k, B, res64(vData), saved, out.nErrors, elapsedBefore + (now() - t0)});
+          }
         } else {
           ++out.nErrors;
-          if (out.nErrors > 3) { throw GpuError("too many Gerbicz errors"); }
+          if (out.nErrors - nErrorsBefore > 3) { throw GpuError("too many Gerbicz errors"); }
           failed = true;
           break;
         }
@@ -614,18 +665,37 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
   return out;
 }
 
-LLResult Gpu::isPrimeLL(u32 iters, std::vector<u64>* resAt, u32 resStep) {
+LLResult Gpu::isPrimeLL(u32 iters, std::vector<u64>* resAt, u32 resStep, Saver<LLState>* saver, u32 saveStep) {
   LLResult out;
   readROE();
   restartRoeWindow();
-  writeIn(DATA, makeWords(E, 4));
+  u32 k0 = 0;
+  double elapsedBefore = 0;
+  if (saver) {
+    LLState s = saver->load();
+    writeIn(DATA, s.data);
+    if (dataResidue() != res64(s.data)) { throw GpuError("invalid LL savefile (res64)"); }
+    k0 = s.k;
+    elapsedBefore = s.elapsed;
+  } else {
+    writeIn(DATA, makeWords(E, 4));
+  }
+  if (resAt && resStep) { resAt->assign(std::min(k0, iters) / resStep, 0); }
   double t0 = now();
   bool leadIn = true;
-  for (u32 k = 1; k <= iters; ++k) {
-    bool leadOut = (resStep && k % resStep == 0) || k == iters || useLongCarry;
+  for (u32 k = k0 + 1; k <= iters; ++k) {
+    const bool save = saver && saveStep && (k % saveStep == 0 || k == iters);
+    bool leadOut = (resStep && k % resStep == 0) || k == iters || save || useLongCarry;
     square(DATA, DATA, leadIn, leadOut, false, true);
     leadIn = leadOut;
     if (resAt && resStep && k % resStep == 0) { resAt->push_back(dataResidue()); }
+    if (save) {
+      Words w = readData(DATA);
+      if (!w.empty()) {  // an all-zero residue is only saved by finishing (src/Gpu.cpp:1626-1642)
+        w.resize(nWords(E), 0);
+        saver->save({E, k, w, elapsedBefore + (now() - t0)});
+      }
+    }
   }
   out.k = iters;
   Words w = readData();

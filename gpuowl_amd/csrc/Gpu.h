@@ -5,6 +5,7 @@
 #include "Buffer.h"
 #include "FFTConfig.h"
 #include "owl_hip.h"
+#include "Saver.h"
 #include "state.h"
 
 #include <hip/hip_vector_types.h>
@@ -113,10 +114,18 @@ public:
   u32 roeCount() const { return roePos; }
   void finish();
 
-  // workflows (src/Gpu.cpp:1337-1663), without savefiles or proofs
+  // workflows (src/Gpu.cpp:1337-1663), without proofs. With a saver, a run starts from the
+  // most recent savefile (src/Gpu.cpp:1169-1191, 1595-1609) and writes one at every verified
+  // Gerbicz check point (PRP, src/Gpu.cpp:1545-1551) or every saveStep iterations (LL,
+  // src/Gpu.cpp:1619-1646); a resumed PRP run keeps the savefile's block size. resAt holds
+  // 0 for the check points before the starting k.
   PRPResult isPrimePRP(u32 iters, u32 blockSize = 1000, u32 checkStep = 0, std::vector<u64>* resAt = nullptr,
-                       u32 resStep = 0);
-  LLResult isPrimeLL(u32 iters, std::vector<u64>* resAt = nullptr, u32 resStep = 0);
+                       u32 resStep = 0, Saver<PRPState>* saver = nullptr);
+  LLResult isPrimeLL(u32 iters, std::vector<u64>* resAt = nullptr, u32 resStep = 0, Saver<LLState>* saver = nullptr,
+                     u32 saveStep = 10'000);
+  // DATA = 3 * check^(2^blockSize - 1) and CHECK = check: x_k from a PRP savefile's check
+  // (src/Gpu.cpp:738-765)
+  void writeState(const Words& check, u32 blockSize);
   double timePRP(u32 iters = 1000, u32 warmup = 30, u32 blockSize = 200);
   double timeSquares(u32 iters, u32 warmup, bool useGraph);
 

@@ -209,6 +209,82 @@ int owl_gpu_ll(owl_gpu g, uint32_t iters, uint64_t* res, uint32_t resStep, uint3
   });
 }
 
+int owl_gpu_prp_saved(owl_gpu g, const char* dir, uint32_t nSavefiles, uint32_t iters, uint32_t blockSize,
+                      uint32_t checkStep, uint64_t* res, uint32_t resStep, uint32_t resCap, owl_prp_result_t* out) {
+  return guarded([&] {
+    if (!dir) { throw std::invalid_argument("savefile directory"); }
+    Saver<PRPState> saver(dir, g->gpu->E, blockSize, nSavefiles);
+    std::vector<u64> at;
+    PRPResult r = g->gpu->isPrimePRP(iters, blockSize, checkStep, res ? &at : nullptr, res ? resStep : 0, &saver);
+    for (size_t i = 0; i < at.size() && i < resCap; ++i) { res[i] = at[i]; }
+    out->res64 = r.res64;
+    out->k = r.k;
+    out->nErrors = r.nErrors;
+    out->nChecks = r.nChecks;
+    out->isPrime = r.isPrime;
+    out->maxRoe = r.maxRoe;
+    out->secsPerIt = r.secsPerIt;
+  });
+}
+
+int owl_gpu_ll_saved(owl_gpu g, const char* dir, uint32_t nSavefiles, uint32_t saveStep, uint32_t iters, uint64_t* res,
+                     uint32_t resStep, uint32_t resCap, owl_ll_result_t* out) {
+  return guarded([&] {
+    if (!dir) { throw std::invalid_argument("savefile directory"); }
+    Saver<LLState> saver(dir, g->gpu->E, 1000, nSavefiles);
+    std::vector<u64> at;
+    LLResult r = g->gpu->isPrimeLL(iters, res ? &at : nullptr, res ? resStep : 0, &saver, saveStep);
+    for (size_t i = 0; i < at.size() && i < resCap; ++i) { res[i] = at[i]; }
+    out->res64 = r.res64;
+    out->k = r.k;
+    out->isPrime = r.isPrime;
+    out->secsPerIt = r.secsPerIt;
+  });
+}
+
+int owl_savefile_write_prp(const char* path, uint32_t E, uint32_t k, uint32_t blockSize, uint64_t res64,
+                           uint32_t nErrors, double elapsed, const uint32_t* check, uint32_t n) {
+  return guarded([&] {
+    if (!path || !check || !E || n != nWords(E)) { throw std::invalid_argument("savefile arguments"); }
+    writePRPState(path, {E, k, blockSize, res64, Words(check, check + n), nErrors, elapsed});
+  });
+}
+
+int owl_savefile_read_prp(const char* path, uint32_t* E, uint32_t* k, uint32_t* blockSize, uint64_t* res64,
+                          uint32_t* nErrors, double* elapsed, uint32_t* check, uint32_t cap) {
+  return guarded([&] {
+    if (!path) { throw std::invalid_argument("savefile path"); }
+    PRPState s = readPRPState(path);
+    if (check && cap < s.check.size()) { throw std::invalid_argument("check buffer too small"); }
+    *E = s.exponent, *k = s.k, *blockSize = s.blockSize, *res64 = s.res64, *nErrors = s.nErrors, *elapsed = s.elapsed;
+    if (check) { std::copy(s.check.begin(), s.check.end(), check); }
+  });
+}
+
+int owl_savefile_write_ll(const char* path, uint32_t E, uint32_t k, double elapsed, const uint32_t* data, uint32_t n) {
+  return guarded([&] {
+    if (!path || !data || !E || n != nWords(E)) { throw std::invalid_argument("savefile arguments"); }
+    writeLLState(path, {E, k, Words(data, data + n), elapsed});
+  });
+}
+
+int owl_savefile_read_ll(const char* path, uint32_t* E, uint32_t* k, double* elapsed, uint32_t* data, uint32_t cap) {
+  return guarded([&] {
+    if (!path) { throw std::invalid_argument("savefile path"); }
+    LLState s = readLLState(path);
+    if (data && cap < s.data.size()) { throw std::invalid_argument("data buffer too small"); }
+    *E = s.exponent, *k = s.k, *elapsed = s.elapsed;
+    if (data) { std::copy(s.data.begin(), s.data.end(), data); }
+  });
+}
+
+int owl_crc32(const void* data, size_t size, uint32_t* crc) {
+  return guarded([&] {
+    if (!data && size) { throw std::invalid_argument("crc32 data"); }
+    *crc = crc32(data, size);
+  });
+}
+
 int owl_gpu_time_prp(owl_gpu g, uint32_t iters, uint32_t warmup, uint32_t blockSize, double* us) {
   return guarded([&] { *us = g->gpu->timePRP(iters, warmup, blockSize); });
 }

@@ -150,6 +150,24 @@ class Gpu:
         check(self.lib.owl_gpu_ll(self.h, iters, res if res_step else None, res_step, cap, C.byref(out)))
         return out, list(res)[:cap]
 
+    def prp_saved(self, directory, iters, block_size=1000, check_step=0, res_step=0, n_savefiles=20):
+        """prp() resumed from / checkpointed to the reference-format savefiles in directory/<E>/."""
+        cap = iters // res_step if res_step else 0
+        res = (C.c_uint64 * max(cap, 1))()
+        out = PrpResult()
+        check(self.lib.owl_gpu_prp_saved(self.h, str(directory).encode(), n_savefiles, iters, block_size, check_step,
+                                         res if res_step else None, res_step, cap, C.byref(out)))
+        return out, list(res)[:cap]
+
+    def ll_saved(self, directory, iters, save_step=10_000, res_step=0, n_savefiles=20):
+        """ll() resumed from / saved to the reference-format savefiles in directory/ll-<E>/."""
+        cap = iters // res_step if res_step else 0
+        res = (C.c_uint64 * max(cap, 1))()
+        out = LLResult()
+        check(self.lib.owl_gpu_ll_saved(self.h, str(directory).encode(), n_savefiles, save_step, iters,
+                                        res if res_step else None, res_step, cap, C.byref(out)))
+        return out, list(res)[:cap]
+
     def time_prp(self, iters=1000, warmup=30, block_size=200):
         us = C.c_double()
         check(self.lib.owl_gpu_time_prp(self.h, iters, warmup, block_size, C.byref(us)))
@@ -366,3 +384,46 @@ def tune_load(path):
 
 def tune_save(path):
     check(load().owl_tune_save(str(path).encode()))
+
+
+# ---------------------------------------------------------------- savefiles (reference formats)
+def crc32(data):
+    """CRC-32/IEEE of a bytes-like object, as the savefiles carry it."""
+    buf = bytes(data)
+    out = C.c_uint32()
+    check(load().owl_crc32(C.c_char_p(buf), len(buf), C.byref(out)))
+    return out.value
+
+
+def savefile_write_prp(path, E, k, block_size, res64, n_errors, elapsed, check_words):
+    w = (C.c_uint32 * len(check_words))(*check_words)
+    check(load().owl_savefile_write_prp(str(path).encode(), E, k, block_size, res64, n_errors, elapsed, w,
+                                        len(check_words)))
+
+
+def savefile_read_prp(path):
+    E, k, b, n = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    r, t = C.c_uint64(), C.c_double()
+    lib = load()
+    p = str(path).encode()
+    check(lib.owl_savefile_read_prp(p, C.byref(E), C.byref(k), C.byref(b), C.byref(r), C.byref(n), C.byref(t), None, 0))
+    w = (C.c_uint32 * n_words(E.value))()
+    check(lib.owl_savefile_read_prp(p, C.byref(E), C.byref(k), C.byref(b), C.byref(r), C.byref(n), C.byref(t), w,
+                                    len(w)))
+    return {"E": E.value, "k": k.value, "block_size": b.value, "res64": r.value, "n_errors": n.value,
+            "elapsed": t.value, "check": list(w)}
+
+
+def savefile_write_ll(path, E, k, elapsed, data_words):
+    w = (C.c_uint32 * len(data_words))(*data_words)
+    check(load().owl_savefile_write_ll(str(path).encode(), E, k, elapsed, w, len(data_words)))
+
+
+def savefile_read_ll(path):
+    E, k, t = C.c_uint32(), C.c_uint32(), C.c_double()
+    lib = load()
+    p = str(path).encode()
+    check(lib.owl_savefile_read_ll(p, C.byref(E), C.byref(k), C.byref(t), None, 0))
+    w = (C.c_uint32 * n_words(E.value))()
+    check(lib.owl_savefile_read_ll(p, C.byref(E), C.byref(k), C.byref(t), w, len(w)))
+    return {"E": E.value, "k": k.value, "elapsed": t.value, "data": list(w)}

@@ -93,6 +93,34 @@ int owl_gpu_arm_fault(owl_gpu gpu, uint32_t k);
 /* Gpu::isPrimeLL (src/Gpu.cpp:1586-1663) without savefiles: iters LL iterations from x0 = 4. */
 int owl_gpu_ll(owl_gpu gpu, uint32_t iters, uint64_t* res, uint32_t resStep, uint32_t resCap, owl_ll_result_t* out);
 
+/* Savefiles in the reference's formats (Saver<PRPState> / Saver<LLState>, src/Saver.cpp:17-30,
+ * 89-143, 160-298; File::writeChecked / readChecked, src/File.h:213-231; CycleFile,
+ * src/CycleFile.cpp:6-21). owl_gpu_prp_saved / owl_gpu_ll_saved are isPrimePRP / isPrimeLL
+ * with a Saver on directory dir: the run starts from the most recent savefile in
+ * dir/<E>/ (PRP) or dir/ll-<E>/ (LL), or from x0 when there is none (a PRP file is accepted
+ * when the res64 of x_k = 3 * check^(2^B - 1) matches its header, else the one before it is
+ * tried, src/Gpu.cpp:1169-1191), writes <E>-<k>.prp at every verified Gerbicz check point
+ * below E (src/Gpu.cpp:1545-1551) or <E>-<k>.ll every saveStep iterations and at the end
+ * (the reference saves LL every 10,000, src/Gpu.cpp:1619-1646), and keeps the newest
+ * nSavefiles. res entries before the starting k are 0. */
+int owl_gpu_prp_saved(owl_gpu gpu, const char* dir, uint32_t nSavefiles, uint32_t iters, uint32_t blockSize,
+                      uint32_t checkStep, uint64_t* res, uint32_t resStep, uint32_t resCap, owl_prp_result_t* out);
+int owl_gpu_ll_saved(owl_gpu gpu, const char* dir, uint32_t nSavefiles, uint32_t saveStep, uint32_t iters, uint64_t* res,
+                     uint32_t resStep, uint32_t resCap, owl_ll_result_t* out);
+/* One savefile: the PRP v13 / LL v13 header line, the CRC-32 of the words, then the
+ * (E-1)/32+1 words (check for PRP, data for LL), written through "<path>.new" and a rename.
+ * Reading also takes PRP v12 and LL v1 files; a bad header, short file or CRC mismatch is an
+ * error. The read functions fill the words when words != NULL (cap >= (E-1)/32+1). */
+int owl_savefile_write_prp(const char* path, uint32_t E, uint32_t k, uint32_t blockSize, uint64_t res64,
+                           uint32_t nErrors, double elapsed, const uint32_t* check, uint32_t nWords);
+int owl_savefile_read_prp(const char* path, uint32_t* E, uint32_t* k, uint32_t* blockSize, uint64_t* res64,
+                          uint32_t* nErrors, double* elapsed, uint32_t* check, uint32_t cap);
+int owl_savefile_write_ll(const char* path, uint32_t E, uint32_t k, double elapsed, const uint32_t* data,
+                          uint32_t nWords);
+int owl_savefile_read_ll(const char* path, uint32_t* E, uint32_t* k, double* elapsed, uint32_t* data, uint32_t cap);
+/* crc32 (src/common.cpp:26-39): CRC-32/IEEE of size bytes. */
+int owl_crc32(const void* data, size_t size, uint32_t* crc);
+
 /* Gpu::timePRP (src/Gpu.cpp:1337-1398): microseconds per PRP iteration incl. Gerbicz updates. */
 int owl_gpu_time_prp(owl_gpu gpu, uint32_t iters, uint32_t warmup, uint32_t blockSize, double* usPerIt);
 /* Steady-state squarings only (no reference counterpart): us per iteration, optionally replayed from a
