@@ -78,6 +78,11 @@ SIGNATURES = {
     "owl_hip_graph_launch": [vp, vp],
     "owl_hip_graph_destroy": [vp],
     "owl_fft_best_fit": [u32, cp, cp, C.c_size_t, P(u32), P(u32)],
+    "owl_fft_tune_load": [cp],
+    "owl_fft_tune_save": [cp],
+    "owl_fft_tune_clear": [],
+    "owl_fft_tune_add": [dbl, cp, P(i32)],
+    "owl_fft_tune_entry": [u32, P(dbl), cp, C.c_size_t, P(u32), P(i32)],
     "owl_gpu_create": [u32, cp, i32, u32, P(vp)],
     "owl_gpu_destroy": [vp],
     "owl_gpu_info": [vp, P(GpuInfo)],

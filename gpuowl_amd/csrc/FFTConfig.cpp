@@ -5,6 +5,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <sstream>
 #include <stdexcept>
 
 namespace {
@@ -194,6 +197,10 @@ double FFTConfig::maxBpw() const {
 
 FFTConfig FFTConfig::bestFit(u32 E, const std::string& spec, double overdrive) {
   if (!spec.empty()) { return FFTConfig{spec}; }
+  // the cheapest tuned FFT that handles E (the list is ordered by cost)
+  for (const FftTuneEntry& e : FftTune::get()) {
+    if (e.fft.maxExp() * overdrive >= E) { return e.fft; }
+  }
   for (const FFTShape& shape : FFTShape::allShapes()) {
     for (u32 v : {101u, 202u}) {
       FFTConfig fft{shape, v, CARRY_AUTO};
@@ -201,4 +208,73 @@ FFTConfig FFTConfig::bestFit(u32 E, const std::string& spec, double overdrive) {
     }
   }
   throw std::invalid_argument("no FFT for exponent " + std::to_string(E));
+}
+
+// ---------------------------------------------------------------- tune file
+namespace {
+std::mutex tuneMutex;
+std::vector<FftTuneEntry>& tuneList() {
+  static std::vector<FftTuneEntry> list;
+  return list;
+}
+}  // namespace
+
+bool FftTune::update(std::vector<FftTuneEntry>& list, const FftTuneEntry& e) {
+  const u32 m = e.fft.maxExp();
+  for (const FftTuneEntry& r : list) {
+    if (r.cost <= e.cost && r.fft.maxExp() >= m) { return false; }  // e is no better anywhere
+  }
+  list.erase(std::remove_if(list.begin(), list.end(),
+                            [&](const FftTuneEntry& r) { return r.cost > e.cost && r.fft.maxExp() <= m; }),
+             list.end());
+  auto at = std::upper_bound(list.begin(), list.end(), e.cost,
+                             [](double c, const FftTuneEntry& r) { return c < r.cost; });
+  list.insert(at, e);
+  return true;
+}
+
+std::vector<FftTuneEntry> FftTune::read(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) { throw std::invalid_argument("cannot read tune file " + path); }
+  std::vector<FftTuneEntry> list;
+  std::string line;
+  while (std::getline(in, line)) {
+    const auto hash = line.find('#');
+    std::istringstream is(line.substr(0, hash));
+    FftTuneEntry e;
+    std::string spec;
+    if (!(is >> e.cost)) { continue; }  // blank or comment-only line
+    if (!(is >> spec)) { throw std::invalid_argument("bad tune line: " + line); }
+    e.fft = FFTConfig{spec};
+    if (!list.empty() && (e.cost < list.back().cost || e.fft.maxExp() <= list.back().fft.maxExp())) {
+      throw std::invalid_argument("tune file not ordered by cost and maxExp at: " + line);
+    }
+    list.push_back(e);
+  }
+  return list;
+}
+
+void FftTune::write(const std::string& path, const std::vector<FftTuneEntry>& list) {
+  const std::string tmp = path + ".new";
+  {
+    std::ofstream out(tmp);
+    if (!out) { throw std::invalid_argument("cannot write tune file " + path); }
+    for (const FftTuneEntry& e : list) {
+      char buf[96];
+      std::snprintf(buf, sizeof buf, "%6.1f %14s # %u\n", e.cost, e.fft.spec().c_str(), e.fft.maxExp());
+      out << buf;
+    }
+    if (!out.flush()) { throw std::invalid_argument("write failed: " + tmp); }
+  }
+  if (std::rename(tmp.c_str(), path.c_str())) { throw std::invalid_argument("cannot rename " + tmp); }
+}
+
+void FftTune::set(std::vector<FftTuneEntry> list) {
+  std::lock_guard<std::mutex> lock(tuneMutex);
+  tuneList() = std::move(list);
+}
+
+std::vector<FftTuneEntry> FftTune::get() {
+  std::lock_guard<std::mutex> lock(tuneMutex);
+  return tuneList();
 }

@@ -49,8 +49,9 @@ enum CarryKind : u32 { CARRY_32 = 0, CARRY_64 = 1, CARRY_AUTO = 2 };
 constexpr u32 LAST_VARIANT = 212;
 
 struct FFTConfig {
-  // spec empty: the first shape (variants 101 then 202) whose maxExp covers E
-  // (src/FFTConfig.cpp:235-263, without a tune file). Otherwise the spec's FFT.
+  // spec empty: the first FFT of the loaded tune file (FftTune) whose maxExp covers E,
+  // else the first shape (variants 101 then 202) that does (src/FFTConfig.cpp:235-263).
+  // Otherwise the spec's FFT.
   static FFTConfig bestFit(u32 E, const std::string& spec, double overdrive = 1.0);
 
   FFTShape shape{};
@@ -66,3 +67,21 @@ struct FFTConfig {
   u32 maxExp() const { return u32(maxBpw() * shape.size()); }
   double maxBpw() const;
 };
+
+// The reference's tune.txt (TuneEntry, src/TuneEntry.cpp:7-84): FFTs ordered by cost with
+// maxExp increasing, so that each entry is the cheapest FFT up to its maxExp. Lines are
+// "cost spec" with an optional "# maxExp" comment; written as "%6.1f %14s # %u".
+struct FftTuneEntry {
+  double cost = 0;
+  FFTConfig fft;
+};
+namespace FftTune {
+// Inserts e unless an entry at most as costly handles at least its maxExp; removes the
+// costlier entries that handle at most e's maxExp. Returns whether the list changed.
+bool update(std::vector<FftTuneEntry>& list, const FftTuneEntry& e);
+std::vector<FftTuneEntry> read(const std::string& path);  // throws on an unreadable or unordered file
+void write(const std::string& path, const std::vector<FftTuneEntry>& list);
+// the process-wide list bestFit consults (empty: no tune file)
+void set(std::vector<FftTuneEntry> list);
+std::vector<FftTuneEntry> get();
+}  // namespace FftTune

@@ -62,6 +62,45 @@ int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen,
   });
 }
 
+int owl_fft_tune_load(const char* path) {
+  return guarded([&] {
+    if (!path) { throw std::invalid_argument("tune file path"); }
+    FftTune::set(FftTune::read(path));
+  });
+}
+
+int owl_fft_tune_save(const char* path) {
+  return guarded([&] {
+    if (!path) { throw std::invalid_argument("tune file path"); }
+    FftTune::write(path, FftTune::get());
+  });
+}
+
+int owl_fft_tune_clear(void) {
+  return guarded([&] { FftTune::set({}); });
+}
+
+int owl_fft_tune_add(double cost, const char* spec, int* updated) {
+  return guarded([&] {
+    if (!spec) { throw std::invalid_argument("spec"); }
+    std::vector<FftTuneEntry> list = FftTune::get();
+    const bool changed = FftTune::update(list, {cost, FFTConfig{spec}});
+    if (changed) { FftTune::set(std::move(list)); }
+    if (updated) { *updated = changed; }
+  });
+}
+
+int owl_fft_tune_entry(uint32_t i, double* cost, char* spec, size_t specLen, uint32_t* maxExp, int* found) {
+  return guarded([&] {
+    std::vector<FftTuneEntry> list = FftTune::get();
+    *found = i < list.size();
+    if (!*found) { return; }
+    if (cost) { *cost = list[i].cost; }
+    if (spec && specLen) { snprintf(spec, specLen, "%s", list[i].fft.spec().c_str()); }
+    if (maxExp) { *maxExp = list[i].fft.maxExp(); }
+  });
+}
+
 int owl_gpu_create(uint32_t E, const char* spec, int flags, uint32_t lineBlock, owl_gpu* out) {
   *out = nullptr;
   return guarded([&] {

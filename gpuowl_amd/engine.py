@@ -427,3 +427,37 @@ def savefile_read_ll(path):
     w = (C.c_uint32 * n_words(E.value))()
     check(lib.owl_savefile_read_ll(p, C.byref(E), C.byref(k), C.byref(t), w, len(w)))
     return {"E": E.value, "k": k.value, "elapsed": t.value, "data": list(w)}
+
+
+# ---------------------------------------------------------------- the reference's tune.txt
+def fft_tune_load(path):
+    """Load a tune.txt ("cost spec [# maxExp]" lines): best_fit(E) then takes its first FFT covering E."""
+    check(load().owl_fft_tune_load(str(path).encode()))
+
+
+def fft_tune_save(path):
+    check(load().owl_fft_tune_save(str(path).encode()))
+
+
+def fft_tune_clear():
+    check(load().owl_fft_tune_clear())
+
+
+def fft_tune_add(cost, spec):
+    """TuneEntry::update: returns whether the list changed."""
+    upd = C.c_int()
+    check(load().owl_fft_tune_add(cost, spec.encode(), C.byref(upd)))
+    return bool(upd.value)
+
+
+def fft_tune_list():
+    """[(cost, spec, maxExp)] in order."""
+    out, lib, i = [], load(), 0
+    while True:
+        cost, mx, found = C.c_double(), C.c_uint32(), C.c_int()
+        spec = C.create_string_buffer(64)
+        check(lib.owl_fft_tune_entry(i, C.byref(cost), spec, 64, C.byref(mx), C.byref(found)))
+        if not found.value:
+            return out
+        out.append((cost.value, spec.value.decode(), mx.value))
+        i += 1

@@ -43,8 +43,20 @@ typedef struct {
   double secsPerIt;
 } owl_ll_result_t;
 
-/* FFTConfig::bestFit (src/FFTConfig.cpp:235-263), without a tune file. spec may be "" or NULL. */
+/* FFTConfig::bestFit (src/FFTConfig.cpp:235-263). spec may be "" or NULL: then the first FFT of the
+ * loaded tune list whose maxExp covers E, else the first shape (variants 101, 202) that does. */
 int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen, uint32_t* N, uint32_t* maxExp);
+/* The reference's tune.txt (TuneEntry, src/TuneEntry.cpp:7-84): a process-wide list of FFTs ordered by
+ * cost (microseconds per iteration) with maxExp increasing, read by bestFit when no spec is given
+ * (owl_fft_best_fit, owl_gpu_create, batches, the server). Lines "cost spec [# maxExp]"; saved as
+ * "%6.1f %14s # %u". load replaces the list (an unordered file is an error); add inserts an entry
+ * unless an entry at most as costly covers at least its maxExp, removing the costlier entries it
+ * covers (*updated = whether the list changed); entry i of the list: *found = 0 past the end. */
+int owl_fft_tune_load(const char* path);
+int owl_fft_tune_save(const char* path);
+int owl_fft_tune_clear(void);
+int owl_fft_tune_add(double cost, const char* spec, int* updated);
+int owl_fft_tune_entry(uint32_t i, double* cost, char* spec, size_t specLen, uint32_t* maxExp, int* found);
 
 /* Gpu::Gpu / Gpu::make (src/Gpu.cpp:380-564). lineBlock 0 = automatic.
  * flags: OWL_LONG_CARRY = full carry every iteration (the reference's -carry long);
