@@ -107,6 +107,10 @@ SIGNATURES = {
     "owl_savefile_write_ll": [cp, u32, u32, dbl, P(u32), u32],
     "owl_savefile_read_ll": [cp, P(u32), P(u32), P(dbl), P(u32), u32],
     "owl_crc32": [vp, C.c_size_t, P(u32)],
+    "owl_worktodo_parse": [cp, P(i32), P(u32), cp, C.c_size_t, P(u32)],
+    "owl_worktodo_best": [P(cp), u32, P(i32)],
+    "owl_prp_type1": [P(u32), u32, u32, P(u64), cp, C.c_size_t],
+    "owl_result_json": [i32, u32, i32, u64, cp, u32, u32, cp, cp, cp, cp, cp, cp, C.c_size_t],
     "owl_gpu_time_prp": [vp, u32, u32, u32, P(dbl)],
     "owl_gpu_time_squares": [vp, u32, u32, i32, P(dbl)],
     "owl_gpu_time_kernels": [vp, u32, P(dbl), P(dbl)],

@@ -5,6 +5,7 @@
 #include "GpuBatch.h"
 #include "Server.h"
 #include "Tune.h"
+#include "Worktodo.h"
 
 #include <deque>
 
@@ -314,6 +315,59 @@ int owl_savefile_read_ll(const char* path, uint32_t* E, uint32_t* k, double* ela
     if (data && cap < s.data.size()) { throw std::invalid_argument("data buffer too small"); }
     *E = s.exponent, *k = s.k, *elapsed = s.elapsed;
     if (data) { std::copy(s.data.begin(), s.data.end(), data); }
+  });
+}
+
+int owl_worktodo_parse(const char* line, int* kind, uint32_t* exponent, char* aid, size_t aidLen, uint32_t* squarings) {
+  return guarded([&] {
+    WorkTask t = parseWorktodoLine(line ? line : "");
+    *kind = t.kind;
+    if (exponent) { *exponent = t.exponent; }
+    if (aid && aidLen) { snprintf(aid, aidLen, "%s", t.aid.c_str()); }
+    if (squarings) { *squarings = t.squarings; }
+  });
+}
+
+int owl_worktodo_best(const char* const* lines, uint32_t n, int32_t* index) {
+  return guarded([&] {
+    std::vector<std::string> v;
+    for (uint32_t i = 0; i < n; ++i) { v.push_back(lines[i] ? lines[i] : ""); }
+    *index = bestWorktodoLine(v);
+  });
+}
+
+int owl_prp_type1(const uint32_t* words, uint32_t n, uint32_t E, uint64_t* r64, char* res2048, size_t len) {
+  return guarded([&] {
+    if (!words || n != nWords(E)) { throw std::invalid_argument("type-1 residue: words"); }
+    Words q = div9(Words(words, words + n), E);
+    if (r64) { *r64 = res64(q); }
+    if (res2048) {
+      if (len < 513) { throw std::invalid_argument("res2048 buffer too small"); }
+      snprintf(res2048, len, "%s", res2048Hex(q).c_str());
+    }
+  });
+}
+
+int owl_result_json(int kind, uint32_t E, int isPrime, uint64_t r64, const char* res2048, uint32_t fftLength,
+                    uint32_t nErrors, const char* aid, const char* user, const char* uid, const char* programName,
+                    const char* programVersion, char* out, size_t outLen) {
+  return guarded([&] {
+    ResultInfo r;
+    r.kind = WorkTask::Kind(kind);
+    r.exponent = E;
+    r.isPrime = isPrime != 0;
+    r.res64 = r64;
+    r.res2048 = res2048 ? res2048 : "";
+    r.fftLength = fftLength;
+    r.nErrors = nErrors;
+    r.aid = aid ? aid : "";
+    r.user = user ? user : "";
+    r.uid = uid ? uid : "";
+    if (programName) { r.programName = programName; }
+    if (programVersion) { r.programVersion = programVersion; }
+    const std::string s = resultJson(r);
+    if (!out || outLen <= s.size()) { throw std::invalid_argument("result buffer too small"); }
+    snprintf(out, outLen, "%s", s.c_str());
   });
 }
 

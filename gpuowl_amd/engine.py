@@ -461,3 +461,44 @@ def fft_tune_list():
             return out
         out.append((cost.value, spec.value.decode(), mx.value))
         i += 1
+
+
+# ---------------------------------------------------------------- worktodo / results (reference formats)
+TASK_KINDS = {0: None, 1: "prp", 2: "ll", 3: "cert"}
+
+
+def worktodo_parse(line):
+    """One worktodo line -> {"kind": "prp"|"ll"|"cert", "exponent", "aid", "squarings"}, or None."""
+    kind, E, sq = C.c_int(), C.c_uint32(), C.c_uint32()
+    aid = C.create_string_buffer(40)
+    check(load().owl_worktodo_parse(line.encode(), C.byref(kind), C.byref(E), aid, 40, C.byref(sq)))
+    k = TASK_KINDS[kind.value]
+    return None if k is None else {"kind": k, "exponent": E.value, "aid": aid.value.decode(), "squarings": sq.value}
+
+
+def worktodo_best(lines):
+    """Index of the task the reference would take first (smallest CERT, else smallest exponent), or -1."""
+    arr = (C.c_char_p * max(1, len(lines)))(*[s.encode() for s in lines])
+    idx = C.c_int32()
+    check(load().owl_worktodo_best(arr, len(lines), C.byref(idx)))
+    return idx.value
+
+
+def prp_type1(words, E):
+    """(res64, res2048) of x * 9^-1 mod 2^E - 1 (the PRP-3 type-1 residue at k = E)."""
+    w = (C.c_uint32 * len(words))(*words)
+    r = C.c_uint64()
+    buf = C.create_string_buffer(520)
+    check(load().owl_prp_type1(w, len(words), E, C.byref(r), buf, 520))
+    return r.value, buf.value.decode()
+
+
+def result_json(kind, E, is_prime, res64, fft_length, res2048="", n_errors=0, aid="", user="", uid="",
+                program_name=None, program_version=None):
+    """One results line in the reference's JSON layout; kind "prp" or "ll"."""
+    k = {"prp": 1, "ll": 2}[kind]
+    buf = C.create_string_buffer(4096)
+    enc = lambda s: s.encode() if s is not None else None  # noqa: E731
+    check(load().owl_result_json(k, E, int(bool(is_prime)), res64, enc(res2048), fft_length, n_errors, enc(aid),
+                                 enc(user), enc(uid), enc(program_name), enc(program_version), buf, 4096))
+    return buf.value.decode()

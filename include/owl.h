@@ -130,6 +130,26 @@ int owl_savefile_read_prp(const char* path, uint32_t* E, uint32_t* k, uint32_t* 
 int owl_savefile_write_ll(const char* path, uint32_t E, uint32_t k, double elapsed, const uint32_t* data,
                           uint32_t nWords);
 int owl_savefile_read_ll(const char* path, uint32_t* E, uint32_t* k, double* elapsed, uint32_t* data, uint32_t cap);
+/* Worktodo lines (src/Worktodo.cpp:25-86): kind OWL_TASK_PRP ("PRP=", "PRPDC="), OWL_TASK_LL ("Test=",
+ * "DoubleCheck="), OWL_TASK_CERT ("Cert=AID,1,2,E,-1,squarings") or OWL_TASK_NONE (comments, unknown
+ * kinds, exponents <= 1000); aid gets the 32-hex-digit assignment id or "". */
+enum { OWL_TASK_NONE = 0, OWL_TASK_PRP = 1, OWL_TASK_LL = 2, OWL_TASK_CERT = 3 };
+int owl_worktodo_parse(const char* line, int* kind, uint32_t* exponent, char* aid, size_t aidLen, uint32_t* squarings);
+/* bestTask (src/Worktodo.cpp:96-106): the index of the smallest CERT, else of the smallest exponent, among
+ * n lines; -1 when none is a task. */
+int owl_worktodo_best(const char* const* lines, uint32_t n, int32_t* index);
+/* The PRP-3 type-1 residue reported at k = E (src/Gpu.cpp:1120-1148, 1495-1504): q = x * 9^-1 mod
+ * 2^E - 1 of the (E-1)/32+1 words x; its res64 and res2048 (the 64 low words as 512 hex digits, most
+ * significant first; len >= 513). */
+int owl_prp_type1(const uint32_t* words, uint32_t nWords, uint32_t E, uint64_t* res64, char* res2048, size_t len);
+/* One results JSON line (src/Task.cpp:86-170) for kind OWL_TASK_PRP ("PRP-3": res64, res2048,
+ * residue-type 1, errors.gerbicz, fft-length) or OWL_TASK_LL (res64, fft-length, shift-count,
+ * error-code), with the program block (programName / programVersion; NULL = this engine's), user,
+ * aid, uid (NULL or "" omitted) and a UTC timestamp. */
+int owl_result_json(int kind, uint32_t E, int isPrime, uint64_t res64, const char* res2048, uint32_t fftLength,
+                    uint32_t nErrors, const char* aid, const char* user, const char* uid, const char* programName,
+                    const char* programVersion, char* out, size_t outLen);
+
 /* crc32 (src/common.cpp:26-39): CRC-32/IEEE of size bytes. */
 int owl_crc32(const void* data, size_t size, uint32_t* crc);
 
