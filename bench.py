@@ -91,6 +91,9 @@ def batch_leg(owl, exps, spec, iters, warmup):
     return {"jobs": len(exps), "groups": groups, "total_words": words, "us_per_batch_iter": round(us_batch, 2),
             "squarings_words_per_s": round(rate), "plain_loop_squarings_words_per_s": round(plain),
             "speedup_vs_plain": round(plain_us / us_batch, 3),
+            # the reference's byte model (64 bytes per word and squaring) over the batch, and the
+            # 32 bytes per word and squaring this engine's two passes move
+            "roofline_frac_64N_model": round(64 * rate / (HBM_PEAK_GBS * 1e9), 4),
             "hbm_frac_32N_model": round(32 * rate / (HBM_PEAK_GBS * 1e9), 4)}
 
 
