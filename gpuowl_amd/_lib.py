@@ -33,7 +33,7 @@ class ServeResult(C.Structure):
     _fields_ = [("id", C.c_uint32), ("E", C.c_uint32), ("iters", C.c_uint32), ("ll", C.c_int),
                 ("res64", C.c_uint64), ("isPrime", C.c_int), ("nErrors", C.c_uint32), ("nChecks", C.c_uint32),
                 ("maxRoe", C.c_double), ("tSubmit", C.c_double), ("tAdmit", C.c_double), ("tDone", C.c_double),
-                ("status", C.c_int)]
+                ("status", C.c_int), ("res64Type1", C.c_uint64), ("res2048", C.c_char * 516)]
 
 
 class ServeInfo(C.Structure):

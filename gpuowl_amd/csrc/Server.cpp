@@ -1,6 +1,8 @@
 // Request loop with continuous batching; see Server.h.
 #include "Server.h"
 
+#include "Worktodo.h"
+
 #include <algorithm>
 #include <chrono>
 #include <stdexcept>
@@ -302,6 +304,12 @@ u32 Server::step() {
         if (j->req.iters == (g->ll ? j->req.E - 2 : j->req.E)) {
           Words w = j->gpu->readData(Gpu::BASE);
           j->res.isPrime = g->ll ? w.empty() : equals9(w);
+          if (!g->ll) {  // the residue results lines report (src/Gpu.cpp:1495-1504)
+            w.resize(nWords(j->req.E), 0);
+            Words q = div9(w, j->req.E);
+            j->res.res64Type1 = res64(q);
+            j->res.res2048 = res2048Hex(q);
+          }
         }
       }
     }

@@ -32,6 +32,8 @@ struct ServeResult {
   double maxRoe = 0;
   double tSubmit = 0, tAdmit = 0, tDone = 0;  // seconds on the server's clock
   int status = 0;         // 0 done, 1 failed (repeated Gerbicz errors)
+  u64 res64Type1 = 0;     // a PRP request run to E: the type-1 residue x_E / 9
+  std::string res2048;    // of the same, 512 hex digits
 };
 
 struct ServeInfo {

@@ -555,6 +555,8 @@ int owl_server_take(owl_server s, owl_serve_result_t* out, uint32_t cap, uint32_
       o.tAdmit = r.tAdmit;
       o.tDone = r.tDone;
       o.status = r.status;
+      o.res64Type1 = r.res64Type1;
+      snprintf(o.res2048, sizeof o.res2048, "%s", r.res2048.c_str());
       s->ready.pop_front();
     }
     *n = i;

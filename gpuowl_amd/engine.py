@@ -309,7 +309,9 @@ class Server:
             n = C.c_uint32()
             check(self.lib.owl_server_take(self.h, buf, 64, C.byref(n)))
             for r in buf[:n.value]:
-                out.append({f: getattr(r, f) for f, _ in ServeResult._fields_})
+                d = {f: getattr(r, f) for f, _ in ServeResult._fields_}
+                d["res2048"] = d["res2048"].decode()
+                out.append(d)
             if n.value < 64:
                 return out
 

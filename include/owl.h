@@ -200,6 +200,10 @@ typedef struct {
   double maxRoe;
   double tSubmit, tAdmit, tDone; /* seconds on the server's clock (owl_server_now) */
   int status;       /* 0 done, 1 failed (repeated Gerbicz errors), 2 rejected */
+  /* a PRP request run to iters == E: the type-1 residue x_E * 9^-1 that results lines report
+   * (res64 and the 64 low words as 512 hex digits, as owl_prp_type1); 0 and "" otherwise */
+  uint64_t res64Type1;
+  char res2048[516];
 } owl_serve_result_t;
 typedef struct {
   uint32_t waiting, active, groups, done, rounds;

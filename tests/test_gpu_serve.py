@@ -1,5 +1,6 @@
 """Serving: requests admitted while others run (continuous batching) end on the same
 res64 as the same request run alone and as GMP; admission respects the memory budget."""
+import json
 import sys
 
 import pytest
@@ -92,13 +93,28 @@ def test_admission_by_memory_budget():
 
 @pytest.mark.slow
 def test_full_mersenne_prime_runs_to_completion():
-    # M859433 is prime: PRP ends on 9 at k = E and LL on 0 at k = E - 2 (long-carry geometry)
+    # M859433 is prime: PRP ends on 9 at k = E and LL on 0 at k = E - 2 (long-carry geometry).
+    # The requests come from worktodo lines and the results go out as results lines: the PRP
+    # one reports the type-1 residue x_E / 9 = 1.
     E = 859_433
+    aid = "0123456789ABCDEF0123456789ABCDEF"
+    lines = [f"PRP={aid},1,2,{E},-1,75,0", f"DoubleCheck={E},75,1"]
     announce("serve full", E)
     with owl.Server(block_size=1000, check_step=50_000) as s:
-        s.submit(E, "prp")
-        s.submit(E, "ll")
+        for line in lines:
+            t = owl.worktodo_parse(line)
+            s.submit(t["exponent"], t["kind"])
         res = s.drain()
     assert len(res) == 2
     for r in res:
         assert r["status"] == 0 and r["isPrime"], r
+    prp = next(r for r in res if not r["ll"])
+    assert prp["res64Type1"] == 1 and prp["res2048"] == "0" * 511 + "1"
+    n = owl.best_fit(E)[1]
+    line = json.loads(owl.result_json("prp", E, prp["isPrime"], prp["res64Type1"], n, res2048=prp["res2048"],
+                                      n_errors=prp["nErrors"], aid=aid))
+    assert (line["status"], line["exponent"], line["worktype"], line["res64"], line["aid"]) == \
+        ("P", E, "PRP-3", "0000000000000001", aid)
+    ll = next(r for r in res if r["ll"])
+    assert ll["res64Type1"] == 0 and ll["res2048"] == ""
+    assert json.loads(owl.result_json("ll", E, ll["isPrime"], ll["res64"], n))["status"] == "P"
