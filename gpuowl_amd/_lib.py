@@ -107,6 +107,11 @@ SIGNATURES = {
     "owl_savefile_write_ll": [cp, u32, u32, dbl, P(u32), u32],
     "owl_savefile_read_ll": [cp, P(u32), P(u32), P(dbl), P(u32), u32],
     "owl_crc32": [vp, C.c_size_t, P(u32)],
+    "owl_proof_points": [u32, u32, P(u32), u32, P(u32)],
+    "owl_gpu_prp_proof": [vp, cp, u32, u32, u32, cp, P(PrpResult)],
+    "owl_gpu_proof_verify": [vp, cp, P(i32), P(i32)],
+    "owl_gpu_exp_mul": [vp, P(u32), u64, P(u32), i32, P(u32), u32],
+    "owl_sha3_256": [vp, C.c_size_t, cp],
     "owl_worktodo_parse": [cp, P(i32), P(u32), cp, C.c_size_t, P(u32)],
     "owl_worktodo_best": [P(cp), u32, P(i32)],
     "owl_prp_type1": [P(u32), u32, u32, P(u64), cp, C.c_size_t],

@@ -9,7 +9,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgpuowl_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["kernels.hip", "kfix_row_a.hip", "kfix_row_b.hip", "kfix_row_c.hip", "kfix_colx_a.hip", "kfix_colx_b.hip", "kfix_colx_c.hip", "kfix_coly_a.hip", "kfix_coly_b.hip", "hipwrap.cpp", "FFTConfig.cpp", "Trig.cpp", "state.cpp", "Saver.cpp", "Worktodo.cpp", "TrigBufCache.cpp", "Gpu.cpp", "GpuBatch.cpp", "Server.cpp", "Tune.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "kfix_row_a.hip", "kfix_row_b.hip", "kfix_row_c.hip", "kfix_colx_a.hip", "kfix_colx_b.hip", "kfix_colx_c.hip", "kfix_coly_a.hip", "kfix_coly_b.hip", "hipwrap.cpp", "FFTConfig.cpp", "Trig.cpp", "state.cpp", "Saver.cpp", "Worktodo.cpp", "Proof.cpp", "TrigBufCache.cpp", "Gpu.cpp", "GpuBatch.cpp", "Server.cpp", "Tune.cpp", "capi.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 # Packed FP32 instructions stay off (no FP32 math on this path anyway).
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -575,7 +575,7 @@ void Gpu::writeState(const Words& check, u32 B) {
 }
 
 PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* resAt, u32 resStep,
-                          Saver<PRPState>* saver) {
+                          Saver<PRPState>* saver, const ProofStore* proof) {
   if (!B) { throw std::invalid_argument("block size"); }
   PRPResult out;
   Words vData = makeWords(E, 3), vCheck = makeWords(E, 1);
@@ -616,6 +616,7 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
     u32 k = vK;
     if (resAt && resStep) { resAt->resize(vK / resStep); }
     bool skip = vK > 0 && vCheckHasX, leadIn = true, failed = false;
+    u32 proofK = proof ? proof->next(k) : ~0u;  // the next iteration whose residue the proof keeps
     while (k < iters) {
       if (skip) {
         skip = false;
@@ -624,10 +625,15 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
       }
       ++k;
       bool fault = (k == faultAt_);
-      bool leadOut = (k % B == 0) || k == iters || (resStep && k % resStep == 0) || useLongCarry || fault;
+      bool leadOut = (k % B == 0) || k == iters || (resStep && k % resStep == 0) || useLongCarry || fault ||
+                     k == proofK;
       square(DATA, DATA, leadIn, leadOut);
       leadIn = leadOut;
       if (fault) { injectFault(); }
+      if (k == proofK) {  // kept at once, as the reference does; a reload past it writes it again
+        proof->save(k, readData(DATA));
+        proofK = proof->next(k);
+      }
       if (resAt && resStep && k % resStep == 0) { resAt->push_back(dataResidue()); }
       bool check = (k % checkStep == 0) || (k == iters && k % B == 0);
       if (check) {
@@ -663,6 +669,88 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
   out.maxRoe = maxRoe;
   out.secsPerIt = (now() - t0) / std::max<u32>(1, iters);
   return out;
+}
+
+Words Gpu::expMul(const Words& A, u64 h, const Words& Bw, bool squareB) {
+  writeIn(CHECK, Bw);
+  if (squareB) { squareLoop(CHECK, 0, 1); }
+  if (h == 0) {
+    writeIn(DATA, makeWords(E, 1));
+  } else {
+    // left-to-right binary: square for every bit below the top one, multiply by A where it is set
+    writeIn(AUX, A);
+    copy(DATA, AUX);
+    u32 run = 0;
+    for (int b = 62 - __builtin_clzll(h) + 1; b-- > 0;) {
+      ++run;
+      if ((h >> b) & 1) {
+        squareLoop(DATA, 0, run);
+        modMul(DATA, AUX);
+        run = 0;
+      }
+    }
+    if (run) { squareLoop(DATA, 0, run); }
+  }
+  modMul(DATA, CHECK);
+  Words r = readData(DATA);
+  r.resize(nWords(E), 0);
+  return r;
+}
+
+Words Gpu::expExp2(const Words& A, u32 n) {
+  writeIn(DATA, A);
+  if (n) { squareLoop(DATA, 0, n); }
+  Words r = readData(DATA);
+  r.resize(nWords(E), 0);
+  return r;
+}
+
+PrpProof Gpu::makeProof(const ProofStore& st) {
+  const std::vector<u32>& pts = st.points();
+  const u32 power = st.power();
+  PrpProof pr;
+  pr.E = E;
+  pr.B = st.load(E);
+  auto hash = proofHash(E, pr.B);
+  std::vector<u64> hashes;
+  for (u32 p = 0; p < power; ++p) {
+    // M_p = prod over the 2^p points at odd multiples of the level's span, each raised to
+    // the hashes of the levels that split it (ProofSet::computeProof, src/Proof.cpp:245-297)
+    std::vector<Words> stack;
+    const u32 s = 1u << (power - p - 1);
+    for (u32 i = 0; i < (1u << p); ++i) {
+      stack.push_back(st.load(pts[s * (2 * i + 1) - 1]));
+      for (u32 k = 0; i & (1u << k); ++k) {
+        Words top = std::move(stack.back());
+        stack.pop_back();
+        stack.back() = expMul(stack.back(), hashes[p - 1 - k], top);
+      }
+    }
+    if (stack.size() != 1) { throw std::logic_error("proof: level did not reduce to one middle"); }
+    if (std::all_of(stack[0].begin(), stack[0].end(), [](u32 w) { return w == 0; })) {
+      throw GpuError("proof: zero middle");
+    }
+    pr.middles.push_back(std::move(stack[0]));
+    hash = proofHash(E, hash, pr.middles.back());
+    hashes.push_back(hash[0]);
+  }
+  return pr;
+}
+
+bool Gpu::verifyProof(const PrpProof& pr) {
+  if (pr.E != E || pr.middles.empty()) { throw std::invalid_argument("proof of another exponent"); }
+  Words A = makeWords(E, 3), Bw = pr.B;
+  Bw.resize(nWords(E), 0);
+  auto hash = proofHash(E, Bw);
+  u32 span = E;
+  for (const Words& M : pr.middles) {
+    hash = proofHash(E, hash, M);
+    const u64 h = hash[0];
+    Bw = expMul(M, h, Bw, span % 2);
+    A = expMul(A, h, M);
+    span = (span + 1) / 2;
+  }
+  return expExp2(A, span) == Bw;
 }
 
 LLResult Gpu::isPrimeLL(u32 iters, std::vector<u64>* resAt, u32 resStep, Saver<LLState>* saver, u32 saveStep) {

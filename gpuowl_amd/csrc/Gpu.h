@@ -5,6 +5,7 @@
 #include "Buffer.h"
 #include "FFTConfig.h"
 #include "owl_hip.h"
+#include "Proof.h"
 #include "Saver.h"
 #include "state.h"
 
@@ -120,7 +121,14 @@ public:
   // src/Gpu.cpp:1619-1646); a resumed PRP run keeps the savefile's block size. resAt holds
   // 0 for the check points before the starting k.
   PRPResult isPrimePRP(u32 iters, u32 blockSize = 1000, u32 checkStep = 0, std::vector<u64>* resAt = nullptr,
-                       u32 resStep = 0, Saver<PRPState>* saver = nullptr);
+                       u32 resStep = 0, Saver<PRPState>* saver = nullptr, const ProofStore* proof = nullptr);
+  // proofs (src/Proof.cpp:84-126, 245-297; src/Gpu.cpp:813-889): A^h * B (B squared first
+  // when squareB) and A^(2^n) on the device; a proof from the residues a full run kept;
+  // the check of a proof (true when A^(2^span) meets B at the end of the hash chain)
+  Words expMul(const Words& A, u64 h, const Words& B, bool squareB = false);
+  Words expExp2(const Words& A, u32 n);
+  PrpProof makeProof(const ProofStore& store);
+  bool verifyProof(const PrpProof& proof);
   LLResult isPrimeLL(u32 iters, std::vector<u64>* resAt = nullptr, u32 resStep = 0, Saver<LLState>* saver = nullptr,
                      u32 saveStep = 10'000);
   // DATA = 3 * check^(2^blockSize - 1) and CHECK = check: x_k from a PRP savefile's check

@@ -1,0 +1,212 @@
+#include "Proof.h"
+
+#include "Saver.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <memory>
+#include <stdexcept>
+
+// ---------------------------------------------------------------- SHA3-256
+namespace {
+
+// Keccak-f[1600] constants derived from their definitions in FIPS 202 (3.2.2, 3.2.5)
+struct KeccakConsts {
+  u64 rc[24];
+  int rot[25];  // by lane index x + 5y
+  KeccakConsts() {
+    auto rcBit = [](int t) {
+      if (t % 255 == 0) { return 1u; }
+      unsigned r = 1;
+      for (int i = 1; i <= t % 255; ++i) {
+        r <<= 1;
+        const unsigned b8 = (r >> 8) & 1;
+        r ^= b8 | (b8 << 4) | (b8 << 5) | (b8 << 6);
+        r &= 0xFF;
+      }
+      return r & 1;
+    };
+    for (int i = 0; i < 24; ++i) {
+      rc[i] = 0;
+      for (int j = 0; j <= 6; ++j) {
+        if (rcBit(j + 7 * i)) { rc[i] |= u64(1) << ((1 << j) - 1); }
+      }
+    }
+    rot[0] = 0;
+    int x = 1, y = 0;
+    for (int t = 0; t < 24; ++t) {
+      rot[x + 5 * y] = ((t + 1) * (t + 2) / 2) % 64;
+      const int nx = y, ny = (2 * x + 3 * y) % 5;
+      x = nx, y = ny;
+    }
+  }
+};
+const KeccakConsts& kc() {
+  static const KeccakConsts c;
+  return c;
+}
+
+inline u64 rotl(u64 v, int n) { return n ? (v << n) | (v >> (64 - n)) : v; }
+
+void keccakF(u64* a) {
+  const KeccakConsts& k = kc();
+  for (int round = 0; round < 24; ++round) {
+    u64 c[5], b[25];
+    for (int x = 0; x < 5; ++x) { c[x] = a[x] ^ a[x + 5] ^ a[x + 10] ^ a[x + 15] ^ a[x + 20]; }
+    for (int x = 0; x < 5; ++x) {
+      const u64 d = c[(x + 4) % 5] ^ rotl(c[(x + 1) % 5], 1);
+      for (int y = 0; y < 5; ++y) { a[x + 5 * y] ^= d; }
+    }
+    for (int x = 0; x < 5; ++x) {  // rho and pi: B[y][2x + 3y] = rot(A[x][y])
+      for (int y = 0; y < 5; ++y) { b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl(a[x + 5 * y], k.rot[x + 5 * y]); }
+    }
+    for (int y = 0; y < 5; ++y) {  // chi
+      for (int x = 0; x < 5; ++x) { a[x + 5 * y] = b[x + 5 * y] ^ (~b[(x + 1) % 5 + 5 * y] & b[(x + 2) % 5 + 5 * y]); }
+    }
+    a[0] ^= k.rc[round];  // iota
+  }
+}
+
+u64 loadLE(const unsigned char* p) {
+  u64 v = 0;
+  for (int i = 7; i >= 0; --i) { v = (v << 8) | p[i]; }
+  return v;
+}
+
+}  // namespace
+
+Sha3_256::Sha3_256() { std::memset(st_, 0, sizeof st_); }
+
+void Sha3_256::absorbBlock(const unsigned char* p) {
+  for (int i = 0; i < 17; ++i) { st_[i] ^= loadLE(p + 8 * i); }
+  keccakF(st_);
+}
+
+Sha3_256& Sha3_256::update(const void* data, size_t n) {
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  while (n) {
+    const size_t take = std::min(n, sizeof buf_ - fill_);
+    std::memcpy(buf_ + fill_, p, take);
+    fill_ += take, p += take, n -= take;
+    if (fill_ == sizeof buf_) {
+      absorbBlock(buf_);
+      fill_ = 0;
+    }
+  }
+  return *this;
+}
+
+std::array<u64, 4> Sha3_256::finish() {
+  std::memset(buf_ + fill_, 0, sizeof buf_ - fill_);
+  buf_[fill_] ^= 0x06;  // SHA-3 domain bits and the first pad bit
+  buf_[sizeof buf_ - 1] ^= 0x80;
+  absorbBlock(buf_);
+  fill_ = 0;
+  return {st_[0], st_[1], st_[2], st_[3]};
+}
+
+std::array<u64, 4> proofHash(u32 E, const Words& w) {
+  if (w.size() * 4 < (E - 1) / 8 + 1) { throw std::invalid_argument("proof hash: words"); }
+  return Sha3_256{}.update(w.data(), (E - 1) / 8 + 1).finish();
+}
+
+std::array<u64, 4> proofHash(u32 E, const std::array<u64, 4>& prefix, const Words& w) {
+  if (w.size() * 4 < (E - 1) / 8 + 1) { throw std::invalid_argument("proof hash: words"); }
+  return Sha3_256{}.update(prefix.data(), 32).update(w.data(), (E - 1) / 8 + 1).finish();
+}
+
+// ---------------------------------------------------------------- points
+std::vector<u32> proofPoints(u32 E, u32 power) {
+  if (!(E & 1) || power < 1 || power > 12) { throw std::invalid_argument("proof power / exponent"); }
+  std::vector<u32> pts{0};
+  for (u32 p = 0, span = (E + 1) / 2; p < power; ++p, span = (span + 1) / 2) {
+    const size_t n = pts.size();
+    for (size_t i = 0; i < n; ++i) { pts.push_back(pts[i] + span); }
+  }
+  pts.front() = E;
+  std::sort(pts.begin(), pts.end());
+  return pts;
+}
+
+// ---------------------------------------------------------------- file
+namespace {
+struct FileCloser {
+  void operator()(FILE* f) const { fclose(f); }
+};
+using FilePtr = std::unique_ptr<FILE, FileCloser>;
+constexpr const char* kHeader = "PRP PROOF\nVERSION=2\nHASHSIZE=64\nPOWER=%u\nNUMBER=M%u\n";
+}  // namespace
+
+void PrpProof::save(const std::string& path) const {
+  const size_t nBytes = (E - 1) / 8 + 1;
+  FilePtr f{fopen(path.c_str(), "wb")};
+  if (!f) { throw std::invalid_argument("cannot write proof " + path); }
+  bool ok = fprintf(f.get(), kHeader, unsigned(middles.size()), E) > 0;
+  auto put = [&](const Words& w) {
+    if (w.size() * 4 < nBytes) { throw std::invalid_argument("proof residue too short"); }
+    ok = ok && fwrite(w.data(), 1, nBytes, f.get()) == nBytes;
+  };
+  put(B);
+  for (const Words& m : middles) { put(m); }
+  if (!ok || fflush(f.get())) { throw std::invalid_argument("proof write failed: " + path); }
+}
+
+PrpProof PrpProof::load(const std::string& path) {
+  FilePtr f{fopen(path.c_str(), "rb")};
+  if (!f) { throw std::invalid_argument("cannot read proof " + path); }
+  unsigned power = 0, E = 0;
+  char nl = 0;
+  if (fscanf(f.get(), "PRP PROOF\nVERSION=2\nHASHSIZE=64\nPOWER=%u\nNUMBER=M%u%c", &power, &E, &nl) != 3 || nl != '\n' ||
+      power < 1 || power > 12 || E < 2) {
+    throw std::invalid_argument("invalid proof header in " + path);
+  }
+  PrpProof p;
+  p.E = E;
+  const size_t nBytes = (E - 1) / 8 + 1;
+  auto get = [&]() {
+    Words w(nWords(E), 0);
+    if (fread(w.data(), 1, nBytes, f.get()) != nBytes) { throw std::invalid_argument("short proof file " + path); }
+    return w;
+  };
+  p.B = get();
+  for (unsigned i = 0; i < power; ++i) { p.middles.push_back(get()); }
+  return p;
+}
+
+// ---------------------------------------------------------------- residue store
+ProofStore::ProofStore(const std::string& dir, u32 E, u32 power)
+    : dir_{dir}, E_{E}, power_{power}, points_{proofPoints(E, power)} {
+  std::filesystem::create_directories(dir_);
+}
+
+u32 ProofStore::next(u32 k) const {
+  auto it = std::upper_bound(points_.begin(), points_.end(), k);
+  return it == points_.end() ? ~0u : *it;
+}
+
+void ProofStore::save(u32 k, const Words& w) const {
+  if (!std::binary_search(points_.begin(), points_.end(), k)) { throw std::invalid_argument("not a proof point"); }
+  Words full = w;
+  full.resize(nWords(E_), 0);
+  const std::string path = dir_ + "/" + std::to_string(k);
+  FilePtr f{fopen(path.c_str(), "wb")};
+  const u32 crc = crc32(full.data(), full.size() * 4);
+  if (!f || fwrite(&crc, 4, 1, f.get()) != 1 || fwrite(full.data(), 4, full.size(), f.get()) != full.size() ||
+      fflush(f.get())) {
+    throw std::invalid_argument("cannot write proof residue " + path);
+  }
+}
+
+Words ProofStore::load(u32 k) const {
+  const std::string path = dir_ + "/" + std::to_string(k);
+  FilePtr f{fopen(path.c_str(), "rb")};
+  u32 crc = 0;
+  Words w(nWords(E_));
+  if (!f || fread(&crc, 4, 1, f.get()) != 1 || fread(w.data(), 4, w.size(), f.get()) != w.size() ||
+      crc32(w.data(), w.size() * 4) != crc) {
+    throw std::invalid_argument("missing or damaged proof residue " + path);
+  }
+  return w;
+}

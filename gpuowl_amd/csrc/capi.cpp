@@ -371,6 +371,62 @@ int owl_result_json(int kind, uint32_t E, int isPrime, uint64_t r64, const char*
   });
 }
 
+int owl_proof_points(uint32_t E, uint32_t power, uint32_t* points, uint32_t cap, uint32_t* n) {
+  return guarded([&] {
+    std::vector<u32> p = proofPoints(E, power);
+    if (points && cap < p.size()) { throw std::invalid_argument("points buffer too small"); }
+    if (points) { std::copy(p.begin(), p.end(), points); }
+    *n = u32(p.size());
+  });
+}
+
+int owl_gpu_prp_proof(owl_gpu g, const char* residueDir, uint32_t power, uint32_t blockSize, uint32_t checkStep,
+                      const char* proofPath, owl_prp_result_t* out) {
+  return guarded([&] {
+    if (!residueDir || !proofPath) { throw std::invalid_argument("proof paths"); }
+    Gpu& G = *g->gpu;
+    ProofStore store(residueDir, G.E, power);
+    PRPResult r = G.isPrimePRP(G.E, blockSize, checkStep, nullptr, 0, nullptr, &store);
+    PrpProof proof = G.makeProof(store);
+    proof.save(proofPath);
+    if (!G.verifyProof(PrpProof::load(proofPath))) { throw GpuError("proof verification failed"); }
+    out->res64 = r.res64;
+    out->k = r.k;
+    out->nErrors = r.nErrors;
+    out->nChecks = r.nChecks;
+    out->isPrime = r.isPrime;
+    out->maxRoe = r.maxRoe;
+    out->secsPerIt = r.secsPerIt;
+  });
+}
+
+int owl_gpu_proof_verify(owl_gpu g, const char* proofPath, int* ok, int* isPrime) {
+  return guarded([&] {
+    if (!proofPath) { throw std::invalid_argument("proof path"); }
+    PrpProof p = PrpProof::load(proofPath);
+    *ok = g->gpu->verifyProof(p);
+    if (isPrime) { *isPrime = equals9(p.B); }
+  });
+}
+
+int owl_gpu_exp_mul(owl_gpu g, const uint32_t* A, uint64_t h, const uint32_t* B, int squareB, uint32_t* out,
+                    uint32_t n) {
+  return guarded([&] {
+    const u32 E = g->gpu->E;
+    if (!A || !B || !out || n != nWords(E)) { throw std::invalid_argument("exp_mul words"); }
+    Words r = g->gpu->expMul(Words(A, A + n), h, Words(B, B + n), squareB != 0);
+    std::copy(r.begin(), r.end(), out);
+  });
+}
+
+int owl_sha3_256(const void* data, size_t size, uint8_t* out) {
+  return guarded([&] {
+    if ((!data && size) || !out) { throw std::invalid_argument("sha3 arguments"); }
+    std::array<u64, 4> d = Sha3_256{}.update(data, size).finish();
+    std::memcpy(out, d.data(), 32);
+  });
+}
+
 int owl_crc32(const void* data, size_t size, uint32_t* crc) {
   return guarded([&] {
     if (!data && size) { throw std::invalid_argument("crc32 data"); }

@@ -150,6 +150,26 @@ class Gpu:
         check(self.lib.owl_gpu_ll(self.h, iters, res if res_step else None, res_step, cap, C.byref(out)))
         return out, list(res)[:cap]
 
+    def prp_proof(self, residue_dir, proof_path, power, block_size=1000, check_step=0):
+        """The full PRP-3 test with a proof of the given power written (and verified) at proof_path."""
+        out = PrpResult()
+        check(self.lib.owl_gpu_prp_proof(self.h, str(residue_dir).encode(), power, block_size, check_step,
+                                         str(proof_path).encode(), C.byref(out)))
+        return out
+
+    def proof_verify(self, proof_path):
+        """(valid, is_prime) of a proof file."""
+        ok, prime = C.c_int(), C.c_int()
+        check(self.lib.owl_gpu_proof_verify(self.h, str(proof_path).encode(), C.byref(ok), C.byref(prime)))
+        return bool(ok.value), bool(prime.value)
+
+    def exp_mul(self, A, h, B, square_b=False):
+        """A^h * B (B squared first if square_b) as words."""
+        n = n_words(self.E)
+        a, b, out = (C.c_uint32 * n)(*A), (C.c_uint32 * n)(*B), (C.c_uint32 * n)()
+        check(self.lib.owl_gpu_exp_mul(self.h, a, h, b, int(square_b), out, n))
+        return list(out)
+
     def prp_saved(self, directory, iters, block_size=1000, check_step=0, res_step=0, n_savefiles=20):
         """prp() resumed from / checkpointed to the reference-format savefiles in directory/<E>/."""
         cap = iters // res_step if res_step else 0
@@ -504,3 +524,21 @@ def result_json(kind, E, is_prime, res64, fft_length, res2048="", n_errors=0, ai
     check(load().owl_result_json(k, E, int(bool(is_prime)), res64, enc(res2048), fft_length, n_errors, enc(aid),
                                  enc(user), enc(uid), enc(program_name), enc(program_version), buf, 4096))
     return buf.value.decode()
+
+
+# ---------------------------------------------------------------- proofs
+def proof_points(E, power):
+    """The iterations whose residues a PRP run keeps for a proof of this power."""
+    lib = load()
+    n = C.c_uint32()
+    check(lib.owl_proof_points(E, power, None, 0, C.byref(n)))
+    buf = (C.c_uint32 * n.value)()
+    check(lib.owl_proof_points(E, power, buf, n.value, C.byref(n)))
+    return list(buf)
+
+
+def sha3_256(data):
+    buf = bytes(data)
+    out = C.create_string_buffer(32)
+    check(load().owl_sha3_256(C.c_char_p(buf), len(buf), out))
+    return out.raw

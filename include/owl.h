@@ -150,6 +150,28 @@ int owl_result_json(int kind, uint32_t E, int isPrime, uint64_t res64, const cha
                     uint32_t nErrors, const char* aid, const char* user, const char* uid, const char* programName,
                     const char* programVersion, char* out, size_t outLen);
 
+/* PRP proofs (Proof / ProofSet, src/Proof.cpp:20-297; Gpu::expMul / expExp2, src/Gpu.cpp:813-889).
+ * owl_gpu_prp_proof runs the full PRP-3 test (E iterations, Gerbicz checks as owl_gpu_prp) keeping the
+ * residues of the 2^power proof points (owl_proof_points) under residueDir, builds the proof and
+ * writes it to proofPath ("PRP PROOF
+VERSION=2
+HASHSIZE=64
+POWER=p
+NUMBER=M<E>
+", then B and the
+ * middles as (E-1)/8+1 little-endian bytes each), then verifies it as the reference does before
+ * keeping one (an invalid proof is an error). owl_gpu_proof_verify checks a proof file:
+ * *ok = 1 when valid, *isPrime = 1 when its B is 9. owl_gpu_exp_mul: out = A^h * B (B squared first
+ * when squareB), all nWords = (E-1)/32+1 words. power: 1..12. */
+int owl_proof_points(uint32_t E, uint32_t power, uint32_t* points, uint32_t cap, uint32_t* n);
+int owl_gpu_prp_proof(owl_gpu gpu, const char* residueDir, uint32_t power, uint32_t blockSize, uint32_t checkStep,
+                      const char* proofPath, owl_prp_result_t* out);
+int owl_gpu_proof_verify(owl_gpu gpu, const char* proofPath, int* ok, int* isPrime);
+int owl_gpu_exp_mul(owl_gpu gpu, const uint32_t* A, uint64_t h, const uint32_t* B, int squareB, uint32_t* out,
+                    uint32_t nWords);
+/* SHA3-256 (FIPS 202) of size bytes, the proof's hash (src/sha3.cpp): 32 bytes to out. */
+int owl_sha3_256(const void* data, size_t size, uint8_t* out);
+
 /* crc32 (src/common.cpp:26-39): CRC-32/IEEE of size bytes. */
 int owl_crc32(const void* data, size_t size, uint32_t* crc);
 
