@@ -606,6 +606,12 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
   if (!checkStep) { checkStep = 10 * B; }
   if (checkStep % B) { throw std::invalid_argument("check step must be a multiple of the block size"); }
   const u32 nErrorsBefore = out.nErrors;
+  // a full test runs on to the next block boundary past E, where it is checked before its
+  // result (x_E, kept on the way) counts (src/Gpu.cpp:1446-1447, 1507)
+  const u32 kEndEnd = (iters == E) ? (E + B - 1) / B * B : iters;
+  u64 finalRes = 0;
+  Words finalWords;
+  bool haveFinal = false;  // x_iters was reached in this run (not when a savefile is past it)
   double t0 = now();
   readROE();
   restartRoeWindow();
@@ -617,7 +623,7 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
     if (resAt && resStep) { resAt->resize(vK / resStep); }
     bool skip = vK > 0 && vCheckHasX, leadIn = true, failed = false;
     u32 proofK = proof ? proof->next(k) : ~0u;  // the next iteration whose residue the proof keeps
-    while (k < iters) {
+    while (k < kEndEnd) {
       if (skip) {
         skip = false;
       } else if (k % B == 0) {
@@ -634,8 +640,13 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
         proof->save(k, readData(DATA));
         proofK = proof->next(k);
       }
-      if (resAt && resStep && k % resStep == 0) { resAt->push_back(dataResidue()); }
-      bool check = (k % checkStep == 0) || (k == iters && k % B == 0);
+      if (k == iters) {
+        haveFinal = true;
+        finalRes = dataResidue();
+        if (iters == E) { finalWords = readData(DATA); }
+      }
+      if (resAt && resStep && k % resStep == 0 && k <= iters) { resAt->push_back(dataResidue()); }
+      bool check = (k % checkStep == 0) || (k == kEndEnd && k % B == 0);
       if (check) {
         maxRoe = std::max(maxRoe, readROE().max);
         Words saved;
@@ -660,12 +671,12 @@ PRPResult Gpu::isPrimePRP(u32 iters, u32 B, u32 checkStep, std::vector<u64>* res
       }
     }
     if (failed) { continue; }
-    out.k = k;
+    out.k = haveFinal ? iters : k;
     break;
   }
   maxRoe = std::max(maxRoe, readROE().max);
-  out.res64 = dataResidue();
-  if (out.k == E) { out.isPrime = equals9(readData()); }
+  out.res64 = haveFinal ? finalRes : dataResidue();
+  if (out.k == E) { out.isPrime = equals9(finalWords); }
   out.maxRoe = maxRoe;
   out.secsPerIt = (now() - t0) / std::max<u32>(1, iters);
   return out;

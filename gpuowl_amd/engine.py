@@ -1,5 +1,6 @@
 """Python mirror of the engine's host interface (the reference's Gpu / FFTConfig,
 src/Gpu.h:213-300, src/FFTConfig.h:69-87) over the C-ABI in include/owl.h."""
+import array
 import ctypes as C
 
 from ._lib import GpuInfo, LLResult, OwlError, PrpResult, ServeInfo, ServeResult, TuneEntry, check, load
@@ -31,11 +32,14 @@ def best_fit(E, spec=""):
 
 
 def words_to_int(words):
-    return sum(int(w) << (32 * i) for i, w in enumerate(words))
+    """The integer of little-endian 32-bit words (linear time, through bytes)."""
+    return int.from_bytes(array.array("I", words).tobytes(), "little")
 
 
 def int_to_words(x, E):
-    return [(x >> (32 * i)) & 0xFFFFFFFF for i in range(n_words(E))]
+    """(E-1)/32+1 little-endian 32-bit words of 0 <= x < 2^(32*n) (linear time, through bytes)."""
+    n = n_words(E)
+    return array.array("I", x.to_bytes(4 * n, "little")).tolist()
 
 
 class Gpu:

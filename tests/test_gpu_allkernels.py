@@ -54,11 +54,9 @@ def test_specialized_kernel_is_exact(name):
     spec, W, R, flags, row, col = cfg
     if name.endswith("m"):
         # the multi-pair column pass (compiled for H >= 2304, one group per CU) runs where a
-        # launch has groups to spare; a batch of 2 jobs gives 2 * W/4 = 512 groups of two pairs
+        # launch has groups to spare: one job already gives W/4 = 256 groups of two pairs
         N = owl.best_fit(1000, spec)[1]
         exps = [workloads.next_prime(15 * N)]
-        while len(exps) < 2:
-            exps.append(workloads.next_prime(exps[-1] + 1))
         announce(name, spec, W, R, exps[0], "x", len(exps))
         owl.tune_set(N, W, R, flags)
         try:
