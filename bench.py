@@ -72,8 +72,19 @@ class PlainLoop:
         return self.us[key]
 
 
-def batch_leg(owl, exps, spec, iters, warmup):
-    """Batched squarings (graph-replayed) vs the plain loop: the same exponents one at a time."""
+def batch_leg(owl, exps, spec, iters, warmup, tune=True):
+    """Batched squarings (graph-replayed) vs the plain loop: the same exponents one at a time.
+    With tune, the batch tuner first picks each FFT length's W x H split for the batch (set-up,
+    outside the timed region, like a tune file prepared beforehand); the plain loop keeps the
+    single-job choices."""
+    tuned = None
+    if tune:
+        t0 = time.time()
+        chosen = {}
+        for r in owl.tune_batch(exps, spec, iters=20):
+            chosen.setdefault(2 * r["W"] * r["H"], r)   # the fastest entry of each length comes first
+        tuned = {"seconds": round(time.time() - t0, 1),
+                 "splits": {str(n): f'{r["W"]}x{r["H"]} {r["row"]}+{r["col"]}' for n, r in sorted(chosen.items())}}
     b = owl.Batch(exps, spec)
     us_batch = b.time_squares(iters, warmup, use_graph=True)
     words = b.total_words
@@ -94,7 +105,8 @@ def batch_leg(owl, exps, spec, iters, warmup):
             # the reference's byte model (64 bytes per word and squaring) over the batch, and the
             # 32 bytes per word and squaring this engine's two passes move
             "roofline_frac_64N_model": round(64 * rate / (HBM_PEAK_GBS * 1e9), 4),
-            "hbm_frac_32N_model": round(32 * rate / (HBM_PEAK_GBS * 1e9), 4)}
+            "hbm_frac_32N_model": round(32 * rate / (HBM_PEAK_GBS * 1e9), 4),
+            "batch_tuning": tuned}
 
 
 def pmc_pass(counter, steps=20, warmup=5):
@@ -201,6 +213,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched legs")
     ap.add_argument("--batch-iters", type=int, default=100)
+    ap.add_argument("--no-batch-tune", action="store_true", help="batched legs with the single-job splits")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sample")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-lambda", type=float, default=4.0)
@@ -300,9 +313,10 @@ def main():
     if not a.no_batch:
         batched = {}
         batched["uniform_64x20M_at_1K:3:256"] = leg(batch_leg, owl, workloads.uniform64_20m(), "1K:3:256",
-                                                    a.batch_iters, 10)
+                                                    a.batch_iters, 10, not a.no_batch_tune)
         note("batch of 64 exponents at 1K:3:256 done")
-        batched["uneven_200_exponents_1M_to_60M"] = leg(batch_leg, owl, workloads.uneven200(), "", a.batch_iters, 10)
+        batched["uneven_200_exponents_1M_to_60M"] = leg(batch_leg, owl, workloads.uneven200(), "", a.batch_iters, 10,
+                                                        not a.no_batch_tune)
         note("batch of 200 uneven exponents done")
 
     served = None

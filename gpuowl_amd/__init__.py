@@ -7,10 +7,10 @@ from ._lib import LIB_PATH, OwlError, load
 from .engine import (AUX, BASE, CHECK, DATA, Batch, Gpu, Server, best_fit, crc32, fft_tune_add, fft_tune_clear, fft_tune_list, fft_tune_load,
                      fft_tune_save, kernel_names, int_to_words, n_words,
                      proof_points, prp_type1, result_json, sha3_256, savefile_read_ll, savefile_read_prp, savefile_write_ll,
-                     savefile_write_prp, worktodo_best, worktodo_parse, tune, tune_clear,
+                     savefile_write_prp, worktodo_best, worktodo_parse, tune, tune_batch, tune_batch_clear, tune_clear,
                      tune_get, tune_load, tune_save, tune_set, words_to_int)
 
-__all__ = ["Gpu", "Batch", "Server", "kernel_names", "tune", "tune_get", "tune_set", "tune_clear", "tune_load", "tune_save", "best_fit", "OwlError", "load", "LIB_PATH", "DATA", "AUX", "CHECK", "BASE", "n_words",
+__all__ = ["Gpu", "Batch", "Server", "kernel_names", "tune", "tune_batch", "tune_batch_clear", "tune_get", "tune_set", "tune_clear", "tune_load", "tune_save", "best_fit", "OwlError", "load", "LIB_PATH", "DATA", "AUX", "CHECK", "BASE", "n_words",
            "words_to_int", "int_to_words", "crc32", "savefile_read_prp", "savefile_write_prp", "savefile_read_ll",
            "savefile_write_ll", "fft_tune_load", "fft_tune_save", "fft_tune_clear", "fft_tune_add", "fft_tune_list", "worktodo_parse",
            "worktodo_best", "prp_type1", "result_json", "proof_points", "sha3_256"]

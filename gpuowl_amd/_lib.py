@@ -132,6 +132,8 @@ SIGNATURES = {
     "owl_batch_time_squares": [vp, u32, u32, i32, P(dbl)],
     "owl_tune_run": [u32, cp, u32, P(TuneEntry), u32, P(u32)],
     "owl_tune_set": [u32, u32, u32, i32, dbl],
+    "owl_tune_batch": [P(u32), u32, cp, u32, P(TuneEntry), u32, P(u32)],
+    "owl_tune_batch_clear": [],
     "owl_tune_get": [u32, P(u32), P(u32), P(i32), P(dbl), P(i32)],
     "owl_tune_clear": [],
     "owl_tune_load": [cp],

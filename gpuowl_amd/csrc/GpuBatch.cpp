@@ -1,5 +1,7 @@
 #include "GpuBatch.h"
 
+#include "Tune.h"
+
 #include <chrono>
 #include <map>
 #include <stdexcept>
@@ -19,7 +21,19 @@ Words wordsOf(u32 E, u32 v) {
 GpuBatch::GpuBatch(const std::vector<u32>& exponents, const std::string& spec, const GpuOptions& opt) {
   std::map<std::string, size_t> byShape;
   for (u32 E : exponents) {
-    jobs.push_back(std::make_unique<Gpu>(E, FFTConfig::bestFit(E, spec), opt));
+    FFTConfig fft = FFTConfig::bestFit(E, spec);
+    GpuOptions o = opt;
+    TuneChoice c;
+    // a split tuned for batches of this FFT length, when the caller chose nothing
+    if (!o.width && !o.lineBlock && !o.generic && !o.colOneRegion && !o.colOnePair &&
+        BatchTuneTable::get(fft.size(), &c)) {
+      o.width = c.width;
+      o.lineBlock = c.lineBlock;
+      o.generic = c.generic;
+      o.colOneRegion = c.colOneRegion;
+      o.colOnePair = c.colOnePair;
+    }
+    jobs.push_back(std::make_unique<Gpu>(E, fft, o));
     Gpu& g = *jobs.back();
     auto [it, fresh] = byShape.emplace(g.shapeKey(), groups.size());
     if (fresh) {

@@ -1,5 +1,7 @@
 #include "Tune.h"
 
+#include "GpuBatch.h"
+
 #include "Trig.h"
 
 #include <algorithm>
@@ -120,4 +122,74 @@ std::vector<TuneResult> tuneShape(u32 E, const std::string& spec, u32 iters, u32
   }
   std::sort(out.begin(), out.end(), [](const TuneResult& a, const TuneResult& b) { return a.choice.us < b.choice.us; });
   return out;
+}
+
+namespace {
+std::mutex batchMtx;
+std::map<u32, TuneChoice>& batchTable() {
+  static std::map<u32, TuneChoice> t;
+  return t;
+}
+}  // namespace
+
+void BatchTuneTable::set(u32 N, const TuneChoice& c) {
+  std::lock_guard<std::mutex> lock(batchMtx);
+  batchTable()[N] = c;
+}
+
+bool BatchTuneTable::get(u32 N, TuneChoice* c) {
+  std::lock_guard<std::mutex> lock(batchMtx);
+  auto it = batchTable().find(N);
+  if (it == batchTable().end()) { return false; }
+  *c = it->second;
+  return true;
+}
+
+void BatchTuneTable::clear() {
+  std::lock_guard<std::mutex> lock(batchMtx);
+  batchTable().clear();
+}
+
+std::vector<TuneResult> tuneBatch(const std::vector<u32>& exps, const std::string& spec, u32 iters, u32 warmup) {
+  std::map<u32, std::vector<u32>> byN;
+  for (u32 E : exps) { byN[FFTConfig::bestFit(E, spec).size()].push_back(E); }
+  std::vector<TuneResult> all;
+  for (auto& [N, es] : byN) {
+    const u32 ND = N / 2;
+    TuneChoice before;
+    const bool had = BatchTuneTable::get(N, &before);
+    std::vector<TuneResult> out;
+    std::set<std::tuple<std::string, std::string, u32, u32, bool>> seen;
+    for (u32 w = 256; w <= 4096; w *= 2) {
+      if (ND % w || ND / w < 64 || ND / w > 4096 || fftPlan(w).empty() || fftPlan(ND / w).empty()) { continue; }
+      TuneChoice c;
+      c.width = w;
+      BatchTuneTable::set(N, c);
+      try {
+        GpuBatch b(es, spec);
+        const Gpu& g = *b.jobs.front();
+        auto key = std::make_tuple(g.rowKernel, g.colKernel, g.W, g.R, g.useLongCarry);
+        if (!seen.insert(key).second) { continue; }
+        TuneResult r;
+        r.choice = c;
+        r.choice.us = b.timeSquares(iters, warmup, true);
+        r.W = g.W, r.H = g.H, r.R = g.R, r.longCarry = g.useLongCarry;
+        r.rowKernel = g.rowKernel, r.colKernel = g.colKernel;
+        out.push_back(r);
+      } catch (const std::invalid_argument&) {
+        // not a valid engine for these exponents at this split
+      }
+    }
+    std::sort(out.begin(), out.end(), [](const TuneResult& a, const TuneResult& b) { return a.choice.us < b.choice.us; });
+    if (!out.empty()) {
+      BatchTuneTable::set(N, out.front().choice);
+    } else if (had) {
+      BatchTuneTable::set(N, before);
+    } else {
+      std::lock_guard<std::mutex> lock(batchMtx);
+      batchTable().erase(N);
+    }
+    all.insert(all.end(), out.begin(), out.end());
+  }
+  return all;
 }

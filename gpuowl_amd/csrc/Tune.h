@@ -27,6 +27,15 @@ void load(const std::string& path);
 void save(const std::string& path);
 }  // namespace TuneTable
 
+// Process-wide table N -> choice for jobs run in batches (GpuBatch), filled by tuneBatch: a
+// batch of many jobs can prefer another split than one job alone (more row groups per CU,
+// e.g. 512 x 1536 instead of 1024 x 768 at N = 1,572,864).
+namespace BatchTuneTable {
+void set(u32 N, const TuneChoice& c);
+bool get(u32 N, TuneChoice* c);
+void clear();
+}  // namespace BatchTuneTable
+
 struct TuneResult {
   TuneChoice choice;
   u32 W = 0, H = 0, R = 0;
@@ -38,3 +47,9 @@ struct TuneResult {
 // length bestFit(E, spec) gives: steady squarings replayed from graphs, iters of them.
 // Fastest first. Candidates that the engine rejects are skipped.
 std::vector<TuneResult> tuneShape(u32 E, const std::string& spec, u32 iters, u32 warmup, int device = 0);
+
+// For every FFT length among the exponents (bestFit(E, spec)), time a batch of those
+// exponents under each W x H split (line block and pass variants by the engine's rules) over
+// iters graph-replayed batch iterations, record the fastest in BatchTuneTable and return all
+// results (us = microseconds per batch iteration), grouped by length, fastest first.
+std::vector<TuneResult> tuneBatch(const std::vector<u32>& exponents, const std::string& spec, u32 iters, u32 warmup);

@@ -517,6 +517,33 @@ int owl_tune_run(uint32_t E, const char* spec, uint32_t iters, owl_tune_entry_t*
   });
 }
 
+int owl_tune_batch(const uint32_t* exponents, uint32_t n, const char* spec, uint32_t iters, owl_tune_entry_t* out,
+                   uint32_t cap, uint32_t* nOut) {
+  return guarded([&] {
+    if (!exponents || !n || !iters) { throw std::invalid_argument("batch tuning arguments"); }
+    std::vector<TuneResult> rs = tuneBatch(std::vector<u32>(exponents, exponents + n), spec ? spec : "", iters,
+                                           std::max<u32>(5, iters / 4));
+    u32 i = 0;
+    for (; i < cap && i < rs.size(); ++i) {
+      const TuneResult& r = rs[i];
+      owl_tune_entry_t& o = out[i];
+      o.width = r.choice.width;
+      o.lineBlock = r.choice.lineBlock;
+      o.flags = 0;
+      o.usPerIter = r.choice.us;
+      o.W = r.W, o.H = r.H, o.R = r.R;
+      o.longCarry = r.longCarry;
+      std::snprintf(o.rowKernel, sizeof o.rowKernel, "%s", r.rowKernel.c_str());
+      std::snprintf(o.colKernel, sizeof o.colKernel, "%s", r.colKernel.c_str());
+    }
+    *nOut = i;
+  });
+}
+
+int owl_tune_batch_clear(void) {
+  return guarded([&] { BatchTuneTable::clear(); });
+}
+
 int owl_tune_set(uint32_t N, uint32_t width, uint32_t lineBlock, int flags, double us) {
   return guarded([&] {
     TuneChoice c;

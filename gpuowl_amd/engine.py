@@ -389,6 +389,24 @@ def tune(E, spec="", iters=200):
             for e in buf[:n.value]]
 
 
+def tune_batch(exponents, spec="", iters=20):
+    """Time batches of these exponents under every W x H split per FFT length and record the fastest
+    for owl_batch_create. Each entry: width, line_block, flags, us (per batch iteration), W, H, R,
+    long_carry, row, col."""
+    lib = load()
+    arr = (C.c_uint32 * len(exponents))(*exponents)
+    buf = (TuneEntry * 256)()
+    n = C.c_uint32()
+    check(lib.owl_tune_batch(arr, len(exponents), (spec or "").encode(), iters, buf, 256, C.byref(n)))
+    return [{"width": e.width, "line_block": e.lineBlock, "flags": e.flags, "us": e.usPerIter, "W": e.W, "H": e.H,
+             "R": e.R, "long_carry": bool(e.longCarry), "row": e.rowKernel.decode(), "col": e.colKernel.decode()}
+            for e in buf[:n.value]]
+
+
+def tune_batch_clear():
+    check(load().owl_tune_batch_clear())
+
+
 def tune_set(N, width, line_block, flags, us=0.0):
     check(load().owl_tune_set(N, width, line_block, flags, us))
 

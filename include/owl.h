@@ -259,6 +259,14 @@ typedef struct {
   char rowKernel[32], colKernel[32];
 } owl_tune_entry_t;
 int owl_tune_run(uint32_t E, const char* spec, uint32_t iters, owl_tune_entry_t* out, uint32_t cap, uint32_t* n);
+/* Batch tuning (no reference counterpart): for each FFT length among the n exponents, time a batch of
+ * them under every W x H split over iters graph-replayed batch iterations (usPerIter = microseconds per
+ * batch iteration) and record the fastest split in a table that owl_batch_create consults when its flags
+ * choose nothing (single jobs keep the owl_tune_* table). Results grouped by length (N = 2 W H),
+ * fastest first. */
+int owl_tune_batch(const uint32_t* exponents, uint32_t n, const char* spec, uint32_t iters, owl_tune_entry_t* out,
+                   uint32_t cap, uint32_t* nOut);
+int owl_tune_batch_clear(void);
 int owl_tune_set(uint32_t N, uint32_t width, uint32_t lineBlock, int flags, double usPerIter);
 /* *found = 0 when no choice is recorded for N */
 int owl_tune_get(uint32_t N, uint32_t* width, uint32_t* lineBlock, int* flags, double* usPerIter, int* found);

@@ -33,3 +33,24 @@ def test_tuned_configurations_are_exact():
             assert got == want, c
     finally:
         owl.tune_clear()
+
+
+def test_batch_tuning_picks_a_split_for_batches_only():
+    exps = workloads.primes_from(3_900_000, 8)
+    N = owl.best_fit(exps[0], "256:2:256")[1]
+    announce("tune batch", exps[0], len(exps))
+    try:
+        res = owl.tune_batch(exps, "256:2:256", iters=10)
+        assert len(res) >= 2 and {2 * r["W"] * r["H"] for r in res} == {N}
+        assert all(a["us"] <= b["us"] for a, b in zip(res, res[1:]))
+        best = res[0]
+        with owl.Batch(exps, "256:2:256") as b:
+            info = b.job_info(0)
+            assert (info["row"], info["col"]) == (best["row"], best["col"])
+            outs, _ = b.prp(200, block_size=100, check_step=100)
+        for E, o in zip(exps[:3], outs[:3]):
+            assert o.res64 == oracle.gmp_res64(E, 200, 200)[0], E
+        with owl.Gpu(exps[0], "256:2:256") as g:   # single jobs keep the reference's split
+            assert g.W == 256
+    finally:
+        owl.tune_batch_clear()

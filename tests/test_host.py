@@ -121,3 +121,12 @@ def test_kernel_names_cover_the_specialized_passes():
     assert len(names) == len(set(names))
     for n in ("rowx1024r4", "rowx256r2", "colx3840", "colx3840m", "coly3840", "coly512", "carryB", "readResidue"):
         assert n in names
+
+
+def test_batch_tuning_rejects_bad_arguments():
+    # validated before any device work
+    from gpuowl_amd import _lib
+    lib = gpuowl_amd.load()
+    n = _lib.C.c_uint32()
+    assert lib.owl_tune_batch(None, 0, b"", 10, None, 0, _lib.C.byref(n)) != 0
+    gpuowl_amd.tune_batch_clear()
