@@ -238,6 +238,8 @@ def main():
         print(f"bench: {msg}", file=sys.stderr, flush=True)
 
     if a.profile_batch:
+        if not a.no_batch_tune:   # the split the batched leg runs (its candidates appear in a trace too)
+            owl.tune_batch(workloads.uniform64_20m(), "1K:3:256", iters=20)
         b = owl.Batch(workloads.uniform64_20m(), "1K:3:256")
         us = b.time_squares(a.steps, a.warmup, use_graph=False)
         print(f"profile batch: {b.n} jobs, {a.warmup}+{a.steps} batch iterations, {us:.1f} us each")
