@@ -94,15 +94,18 @@ int owl_gpu_copy(owl_gpu gpu, int dst, int src);
 /* Gpu::readROE (src/Gpu.cpp:624-637): statistics of the max round-off per iteration since the last read. */
 int owl_gpu_read_roe(owl_gpu gpu, uint32_t* n, double* maxRoe, double* mean, double* sd);
 
-/* Gpu::isPrimePRP (src/Gpu.cpp:1400-1584) without savefiles or proofs: iters PRP-3 iterations
- * from x0 = 3 with a Gerbicz check every checkStep (multiple of blockSize). res64 after every
- * resStep iterations goes to res[0..resCap). */
+/* Gpu::isPrimePRP (src/Gpu.cpp:1400-1584) without savefiles or proofs (owl_gpu_prp_saved and
+ * owl_gpu_prp_proof below add them): iters PRP-3 iterations from x0 = 3 with a Gerbicz check every
+ * checkStep (multiple of blockSize). A full test (iters == E) runs on to the next multiple of
+ * blockSize and is checked there; res64 and isPrime are those of x_E. res64 after every resStep
+ * iterations goes to res[0..resCap). */
 int owl_gpu_prp(owl_gpu gpu, uint32_t iters, uint32_t blockSize, uint32_t checkStep, uint64_t* res, uint32_t resStep,
                 uint32_t resCap, owl_prp_result_t* out);
 /* Failure-handling tests (the reference tests its recovery by hand, src/Gpu.cpp:1561-1575):
    the next PRP run flips one data bit right after iteration k, once. 0 disarms. */
 int owl_gpu_arm_fault(owl_gpu gpu, uint32_t k);
-/* Gpu::isPrimeLL (src/Gpu.cpp:1586-1663) without savefiles: iters LL iterations from x0 = 4. */
+/* Gpu::isPrimeLL (src/Gpu.cpp:1586-1663) without savefiles (owl_gpu_ll_saved adds them): iters LL
+ * iterations from x0 = 4. */
 int owl_gpu_ll(owl_gpu gpu, uint32_t iters, uint64_t* res, uint32_t resStep, uint32_t resCap, owl_ll_result_t* out);
 
 /* Savefiles in the reference's formats (Saver<PRPState> / Saver<LLState>, src/Saver.cpp:17-30,
