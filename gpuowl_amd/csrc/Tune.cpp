@@ -178,6 +178,15 @@ std::vector<TuneResult> tuneBatch(const std::vector<u32>& exps, const std::strin
         out.push_back(r);
       } catch (const std::invalid_argument&) {
         // not a valid engine for these exponents at this split
+      } catch (...) {
+        // any other failure (device memory, a kernel error): the table keeps what it had
+        if (had) {
+          BatchTuneTable::set(N, before);
+        } else {
+          std::lock_guard<std::mutex> lock(batchMtx);
+          batchTable().erase(N);
+        }
+        throw;
       }
     }
     std::sort(out.begin(), out.end(), [](const TuneResult& a, const TuneResult& b) { return a.choice.us < b.choice.us; });
