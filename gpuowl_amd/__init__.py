@@ -6,7 +6,7 @@ loads it (ctypes) and mirrors the reference's Gpu / FFTConfig interface."""
 from ._lib import LIB_PATH, OwlError, load
 from .engine import (AUX, BASE, CHECK, DATA, Batch, Gpu, Server, best_fit, crc32, fft_tune_add, fft_tune_clear, fft_tune_list, fft_tune_load,
                      fft_tune_save, kernel_names, int_to_words, n_words,
-                     proof_points, prp_type1, result_json, sha3_256, savefile_read_ll, savefile_read_prp, savefile_write_ll,
+                     proof_md5, proof_points, prp_type1, result_json, root1, sha3_256, savefile_read_ll, savefile_read_prp, savefile_write_ll,
                      savefile_write_prp, worktodo_best, worktodo_parse, tune, tune_batch, tune_batch_clear, tune_clear,
                      tune_get, tune_load, tune_save, tune_set, words_to_int)
 

@@ -42,6 +42,15 @@ class ServeInfo(C.Structure):
                 ("iterations", C.c_uint64), ("wordIterations", C.c_uint64)]
 
 
+class ResultRec(C.Structure):
+    """owl_result_t (include/owl.h)."""
+    _fields_ = [("kind", C.c_int), ("exponent", C.c_uint32), ("isPrime", C.c_int), ("res64", C.c_uint64),
+                ("res2048", C.c_char_p), ("fftLength", C.c_uint32), ("nErrors", C.c_uint32),
+                ("proofPower", C.c_uint32), ("proofMd5", C.c_char_p), ("sha3Hash", C.c_char_p),
+                ("squarings", C.c_uint32), ("aid", C.c_char_p), ("user", C.c_char_p), ("uid", C.c_char_p),
+                ("programName", C.c_char_p), ("programVersion", C.c_char_p)]
+
+
 class TuneEntry(C.Structure):
     _fields_ = [("width", C.c_uint32), ("lineBlock", C.c_uint32), ("flags", C.c_int), ("usPerIter", C.c_double),
                 ("W", C.c_uint32), ("H", C.c_uint32), ("R", C.c_uint32), ("longCarry", C.c_int),
@@ -107,6 +116,7 @@ SIGNATURES = {
     "owl_savefile_write_ll": [cp, u32, u32, dbl, P(u32), u32],
     "owl_savefile_read_ll": [cp, P(u32), P(u32), P(dbl), P(u32), u32],
     "owl_crc32": [vp, C.c_size_t, P(u32)],
+    "owl_trig_root1": [u32, u64, P(dbl), P(dbl)],
     "owl_proof_points": [u32, u32, P(u32), u32, P(u32)],
     "owl_gpu_prp_proof": [vp, cp, u32, u32, u32, cp, P(PrpResult)],
     "owl_gpu_proof_verify": [vp, cp, P(i32), P(i32)],
@@ -116,6 +126,9 @@ SIGNATURES = {
     "owl_worktodo_best": [P(cp), u32, P(i32)],
     "owl_prp_type1": [P(u32), u32, u32, P(u64), cp, C.c_size_t],
     "owl_result_json": [i32, u32, i32, u64, cp, u32, u32, cp, cp, cp, cp, cp, cp, C.c_size_t],
+    "owl_result_json_ex": [P(ResultRec), cp, C.c_size_t],
+    "owl_proof_md5": [cp, cp, C.c_size_t],
+    "owl_gpu_cert": [vp, cp, u32, i32, cp, C.c_size_t, P(u64)],
     "owl_gpu_time_prp": [vp, u32, u32, u32, P(dbl)],
     "owl_gpu_time_squares": [vp, u32, u32, i32, P(dbl)],
     "owl_gpu_time_kernels": [vp, u32, P(dbl), P(dbl)],
@@ -141,6 +154,7 @@ SIGNATURES = {
     "owl_server_create": [u64, u32, u32, u32, cp, i32, P(vp)],
     "owl_server_destroy": [vp],
     "owl_server_submit": [vp, u32, i32, u32, P(u32)],
+    "owl_server_arm_fault": [vp, u32, u32],
     "owl_server_step": [vp, P(u32)],
     "owl_server_take": [vp, P(ServeResult), u32, P(u32)],
     "owl_server_info": [vp, P(ServeInfo)],

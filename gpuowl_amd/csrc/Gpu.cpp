@@ -796,12 +796,28 @@ LLResult Gpu::isPrimeLL(u32 iters, std::vector<u64>* resAt, u32 resStep, Saver<L
       }
     }
   }
-  out.k = iters;
+  // a savefile already past iters (left by a longer run) is reported as what it holds: x_k0
+  out.k = std::max(k0, iters);
   Words w = readData();
   out.res64 = res64(w);
-  out.isPrime = (iters == E - 2) && w.empty();
+  out.isPrime = (out.k == E - 2) && w.empty();
   out.secsPerIt = (now() - t0) / std::max<u32>(1, iters);
   return out;
+}
+
+Words Gpu::isCERT(const Words& start, u32 squarings) {
+  if (!squarings) { throw std::invalid_argument("CERT: no squarings"); }
+  Words x = start;
+  x.resize(nWords(E), 0);
+  // only the E low bits are the start value (the file's last byte may carry more)
+  if (E % 32) { x.back() &= (1u << (E % 32)) - 1; }
+  readROE();
+  restartRoeWindow(0);  // the reference measures no round-off for CERT (wantROE = 0)
+  writeIn(DATA, x);
+  squareLoop(DATA, 0, squarings);
+  Words r = readData(DATA);
+  r.resize(nWords(E), 0);
+  return r;
 }
 
 double Gpu::timePRP(u32 iters, u32 warmup, u32 B) {

@@ -131,6 +131,9 @@ public:
   bool verifyProof(const PrpProof& proof);
   LLResult isPrimeLL(u32 iters, std::vector<u64>* resAt = nullptr, u32 resStep = 0, Saver<LLState>* saver = nullptr,
                      u32 saveStep = 10'000);
+  // CERT (src/Gpu.cpp:1665-1725): `squarings` plain squarings of the start value (the M<E>.cert
+  // file's (E-1)/8+1 little-endian bytes as words); the final residue, (E-1)/32+1 words.
+  Words isCERT(const Words& start, u32 squarings);
   // DATA = 3 * check^(2^blockSize - 1) and CHECK = check: x_k from a PRP savefile's check
   // (src/Gpu.cpp:738-765)
   void writeState(const Words& check, u32 blockSize);

@@ -95,6 +95,7 @@ std::vector<u64> GpuBatch::res64(Gpu::Buf b) {
 std::vector<PRPResult> GpuBatch::prp(u32 iters, u32 B, u32 checkStep, std::vector<std::vector<u64>>* resAt,
                                      u32 resStep) {
   if (!B) { throw std::invalid_argument("block size"); }
+  if (!iters) { throw std::invalid_argument("a batch PRP run needs at least one iteration"); }
   if (!checkStep) { checkStep = 10 * B; }
   if (checkStep % B) { throw std::invalid_argument("check step must be a multiple of the block size"); }
   const size_t n = size();
@@ -112,6 +113,14 @@ std::vector<PRPResult> GpuBatch::prp(u32 iters, u32 B, u32 checkStep, std::vecto
     for (size_t i : g.index) { jobs[i]->readROE(cnt); }
     g.lead->restartRoeWindow();
   }
+  // A full test of one of the jobs (iters == E) runs on to the next block boundary past E,
+  // where it is Gerbicz-checked before x_E (kept on the way) counts, as Gpu::isPrimePRP does
+  // (src/Gpu.cpp:1446-1447, 1507); the other jobs report x_iters the same way.
+  bool full = false;
+  for (auto& j : jobs) { full = full || j->E == iters; }
+  const u32 kEndEnd = full ? (iters + B - 1) / B * B : iters;
+  std::vector<u64> finalRes(n);
+  std::vector<char> finalNine(n, 0);
   double t0 = now();
   // round-off slots advance on each group's lead; every job of the group reads that many
   auto readRoe = [&](std::vector<PRPResult>& res) {
@@ -131,7 +140,7 @@ std::vector<PRPResult> GpuBatch::prp(u32 iters, u32 B, u32 checkStep, std::vecto
     // a verified state already holds x_vK in the check (src/Gpu.cpp:1466-1471, 1545)
     bool skip = vK > 0, failed = false;
     std::vector<bool> leadIn(groups.size(), true);
-    while (k < iters) {
+    while (k < kEndEnd) {
       if (skip) {
         skip = false;
       } else if (k % B == 0) {
@@ -140,7 +149,7 @@ std::vector<PRPResult> GpuBatch::prp(u32 iters, u32 B, u32 checkStep, std::vecto
       ++k;
       bool fault = false;
       for (auto& j : jobs) { fault = fault || j->faultAt() == k; }
-      bool leadOut = (k % B == 0) || k == iters || (resStep && k % resStep == 0) || fault;
+      bool leadOut = (k % B == 0) || k == iters || k == kEndEnd || (resStep && k % resStep == 0) || fault;
       for (size_t gi = 0; gi < groups.size(); ++gi) {
         Group& g = groups[gi];
         bool lo = leadOut || g.lead->useLongCarry;
@@ -153,11 +162,17 @@ std::vector<PRPResult> GpuBatch::prp(u32 iters, u32 B, u32 checkStep, std::vecto
           if (j->faultAt() == k) { j->injectFault(); }
         }
       }
-      if (resAt && resStep && k % resStep == 0) {
+      if (k == iters) {
+        finalRes = res64(Gpu::DATA);
+        for (size_t i = 0; i < n; ++i) {
+          finalNine[i] = (jobs[i]->E == iters) && equals9(jobs[i]->readData(Gpu::DATA));
+        }
+      }
+      if (resAt && resStep && k % resStep == 0 && k <= iters) {
         std::vector<u64> r = res64(Gpu::DATA);
         for (size_t i = 0; i < n; ++i) { (*resAt)[i].push_back(r[i]); }
       }
-      if ((k % checkStep == 0) || (k == iters && k % B == 0)) {
+      if ((k % checkStep == 0) || (k == kEndEnd && k % B == 0)) {
         std::vector<bool> ok = doCheck(B);
         bool all = true;
         for (size_t i = 0; i < n; ++i) {
@@ -187,9 +202,9 @@ std::vector<PRPResult> GpuBatch::prp(u32 iters, u32 B, u32 checkStep, std::vecto
     readRoe(out);
     double secs = (now() - t0) / std::max<u32>(1, iters);
     for (size_t i = 0; i < n; ++i) {
-      out[i].k = k;
-      out[i].res64 = jobs[i]->dataResidue(Gpu::DATA);
-      out[i].isPrime = (k == jobs[i]->E) && equals9(jobs[i]->readData(Gpu::DATA));
+      out[i].k = iters;
+      out[i].res64 = finalRes[i];
+      out[i].isPrime = finalNine[i];
       out[i].secsPerIt = secs;
     }
     return out;

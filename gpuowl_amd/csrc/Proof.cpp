@@ -3,6 +3,7 @@
 #include "Saver.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
@@ -173,6 +174,98 @@ PrpProof PrpProof::load(const std::string& path) {
   p.B = get();
   for (unsigned i = 0; i < power; ++i) { p.middles.push_back(get()); }
   return p;
+}
+
+// ---------------------------------------------------------------- MD5
+// RFC 1321. The additive constants are floor(|sin(i + 1)| * 2^32) (section 3.4), computed once
+// in long double; the per-round shift amounts are the RFC's four rows of four.
+namespace {
+struct Md5Consts {
+  u32 T[64];
+  Md5Consts() {
+    for (int i = 0; i < 64; ++i) { T[i] = u32(std::floor(std::fabs(std::sin((long double)(i + 1))) * 4294967296.0L)); }
+  }
+};
+inline u32 rotl32(u32 v, int n) { return (v << n) | (v >> (32 - n)); }
+
+void md5Block(u32* h, const unsigned char* p) {
+  static const Md5Consts c;
+  static const int S[4][4] = {{7, 12, 17, 22}, {5, 9, 14, 20}, {4, 11, 16, 23}, {6, 10, 15, 21}};
+  u32 X[16];
+  for (int i = 0; i < 16; ++i) { X[i] = u32(p[4 * i]) | u32(p[4 * i + 1]) << 8 | u32(p[4 * i + 2]) << 16 | u32(p[4 * i + 3]) << 24; }
+  u32 a = h[0], b = h[1], cc = h[2], d = h[3];
+  for (int i = 0; i < 64; ++i) {
+    const int round = i / 16;
+    u32 f;
+    int k;
+    switch (round) {
+      case 0: f = (b & cc) | (~b & d), k = i; break;
+      case 1: f = (b & d) | (cc & ~d), k = (5 * i + 1) % 16; break;
+      case 2: f = b ^ cc ^ d, k = (3 * i + 5) % 16; break;
+      default: f = cc ^ (b | ~d), k = (7 * i) % 16; break;
+    }
+    const u32 t = d;
+    d = cc;
+    cc = b;
+    b = b + rotl32(a + f + c.T[i] + X[k], S[round][i % 4]);
+    a = t;
+  }
+  h[0] += a, h[1] += b, h[2] += cc, h[3] += d;
+}
+
+struct Md5 {
+  u32 h[4] = {0x67452301, 0xefcdab89, 0x98badcfe, 0x10325476};
+  unsigned char buf[64];
+  size_t fill = 0;
+  u64 total = 0;
+  void update(const unsigned char* p, size_t n) {
+    total += n;
+    while (n) {
+      const size_t take = std::min(n, 64 - fill);
+      std::memcpy(buf + fill, p, take);
+      fill += take, p += take, n -= take;
+      if (fill == 64) {
+        md5Block(h, buf);
+        fill = 0;
+      }
+    }
+  }
+  std::string hex() {
+    const u64 bits = total * 8;
+    const unsigned char one = 0x80, zero = 0;
+    update(&one, 1);
+    while (fill != 56) { update(&zero, 1); }
+    unsigned char len[8];
+    for (int i = 0; i < 8; ++i) { len[i] = (unsigned char)(bits >> (8 * i)); }
+    update(len, 8);
+    static const char* digits = "0123456789abcdef";
+    std::string s;
+    for (u32 w : h) {
+      for (int i = 0; i < 4; ++i) {
+        const unsigned byte = (w >> (8 * i)) & 0xFF;
+        s += digits[byte >> 4];
+        s += digits[byte & 15];
+      }
+    }
+    return s;
+  }
+};
+}  // namespace
+
+std::string md5Hex(const void* data, size_t n) {
+  Md5 m;
+  m.update(static_cast<const unsigned char*>(data), n);
+  return m.hex();
+}
+
+std::string md5File(const std::string& path) {
+  FilePtr f{fopen(path.c_str(), "rb")};
+  if (!f) { throw std::invalid_argument("cannot read " + path); }
+  Md5 m;
+  unsigned char buf[65536];
+  size_t got = 0;
+  while ((got = fread(buf, 1, sizeof buf, f.get())) > 0) { m.update(buf, got); }
+  return m.hex();
 }
 
 // ---------------------------------------------------------------- residue store

@@ -26,6 +26,11 @@ private:
   void absorbBlock(const unsigned char* p);
 };
 
+// MD5 (RFC 1321) as 32 lowercase hex digits; of a whole file for the "proof" object of a PRP
+// results line (proof::fileHash, src/Proof.cpp:32-39; MD5Hash::finish, src/MD5.h:21-31)
+std::string md5Hex(const void* data, size_t n);
+std::string md5File(const std::string& path);
+
 // hash of the first (E-1)/8+1 bytes of the words, optionally after a previous digest
 std::array<u64, 4> proofHash(u32 E, const Words& w);
 std::array<u64, 4> proofHash(u32 E, const std::array<u64, 4>& prefix, const Words& w);

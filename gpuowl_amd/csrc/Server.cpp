@@ -51,6 +51,13 @@ u32 Server::submit(u32 E, bool ll, u32 iters) {
   if (double(E) / fft.size() < FFTShape::MIN_BPW || E % 32 == 0 || E < 64) {
     throw std::invalid_argument("exponent " + std::to_string(E) + " is too small for FFT " + fft.spec());
   }
+  // a request that could never be admitted would wait forever: reject it now
+  const u64 need = Gpu::footprintBound(fft.size(), fft.shape.width, fft.shape.middle * fft.shape.height);
+  if (need > opt_.memBudget) {
+    throw std::invalid_argument("exponent " + std::to_string(E) + " needs " + std::to_string(need) +
+                                " device bytes, more than the server's memory budget of " +
+                                std::to_string(opt_.memBudget));
+  }
   u32 id = nextId_++;
   waiting_.push_back({id, E, iters, ll, now()});
   return id;
@@ -95,6 +102,10 @@ void Server::admit() {
       continue;
     }
     Gpu& g = *job->gpu;
+    if (auto f = faults_.find(r.id); f != faults_.end()) {
+      job->faultAt = f->second;
+      faults_.erase(f);
+    }
     job->bytes = g.deviceBytes();
     job->kEndEnd = (r.iters + opt_.blockSize - 1) / opt_.blockSize * opt_.blockSize;
     if (r.ll) {
@@ -295,6 +306,16 @@ u32 Server::step() {
   }
   for (Group* g : gs) { g->lead->finish(); }
 
+  // armed faults: one bit of the data flipped after the round that passed iteration faultAt
+  for (Group* g : gs) {
+    for (Job* j : g->members) {
+      if (j->faultAt && j->k >= j->faultAt) {
+        j->faultAt = 0;
+        j->gpu->injectFault();
+      }
+    }
+  }
+
   // residues taken in this round
   for (Group* g : gs) {
     for (Job* j : g->members) {
@@ -356,6 +377,14 @@ u32 Server::step() {
   retire();
   ++rounds_;
   return nDone_ - before;
+}
+
+void Server::armFault(u32 id, u32 k) {
+  auto it = std::find_if(waiting_.begin(), waiting_.end(), [&](const Request& r) { return r.id == id; });
+  if (it == waiting_.end()) { throw std::invalid_argument("armFault: no waiting request with that id"); }
+  if (it->ll) { throw std::invalid_argument("armFault: LL requests have no error check"); }
+  if (!k || k >= it->iters) { throw std::invalid_argument("armFault: the iteration must lie inside the request"); }
+  faults_[id] = k;
 }
 
 std::vector<ServeResult> Server::take() {

@@ -65,6 +65,10 @@ public:
   u32 step();
   // Finished results since the last call.
   std::vector<ServeResult> take();
+  // Failure-handling tests: flip one data bit of request `id` (a PRP request not admitted yet)
+  // at the end of the round in which it passes iteration k, once, as a hardware error would;
+  // its next Gerbicz check must catch it and reload its last verified state.
+  void armFault(u32 id, u32 k);
   ServeInfo info() const;
   double now() const;
 
@@ -79,6 +83,7 @@ private:
     std::unique_ptr<Gpu> gpu;
     u64 bytes = 0;
     u32 k = 0, kEndEnd = 0, vK = 0;
+    u32 faultAt = 0;  // armFault: inject after the round that passes this iteration (0 = none)
     bool skip = false, captured = false, resRead = false, wantCheck = false;
     Words vData, vCheck;
     ServeResult res;
@@ -100,6 +105,7 @@ private:
   double t0_;
   u32 nextId_ = 1;
   std::deque<Request> waiting_;
+  std::map<u32, u32> faults_;  // armFault requests by id, taken at admission
   std::vector<std::unique_ptr<Job>> active_;
   std::map<std::string, std::unique_ptr<Group>> groups_;
   std::vector<ServeResult> done_;

@@ -179,22 +179,30 @@ std::string hex16(u64 x) {
 }  // namespace
 
 std::string resultJson(const ResultInfo& r) {
-  if (r.kind != WorkTask::PRP && r.kind != WorkTask::LL) { throw std::invalid_argument("result kind"); }
-  const bool prp = r.kind == WorkTask::PRP;
-  std::vector<std::string> f{field("status", str(r.isPrime ? "P" : "C")),
-                             field("exponent", std::to_string(r.exponent)),
-                             field("worktype", str(prp ? "PRP-3" : "LL"))};
-  if (prp) {
-    f.push_back(field("res64", str(hex16(r.res64))));
-    f.push_back(field("res2048", str(r.res2048)));
-    f.push_back(field("residue-type", "1"));
-    f.push_back(field("errors", object({field("gerbicz", std::to_string(r.nErrors))})));
-    f.push_back(field("fft-length", std::to_string(r.fftLength)));
+  if (r.kind != WorkTask::PRP && r.kind != WorkTask::LL && r.kind != WorkTask::CERT) {
+    throw std::invalid_argument("result kind");
+  }
+  std::vector<std::string> f;
+  if (r.kind == WorkTask::CERT) {
+    if (r.sha3Hash.size() != 64) { throw std::invalid_argument("CERT result: sha3-hash must be 64 hex digits"); }
+    f = {field("worktype", str("Cert")), field("exponent", std::to_string(r.exponent)),
+         field("sha3-hash", str(r.sha3Hash)), field("squarings", std::to_string(r.squarings)),
+         field("fft-length", std::to_string(r.fftLength)), field("shift-count", "0"),
+         field("error-code", str("00000000"))};
+  } else if (r.kind == WorkTask::PRP) {
+    f = {field("status", str(r.isPrime ? "P" : "C")), field("exponent", std::to_string(r.exponent)),
+         field("worktype", str("PRP-3")), field("res64", str(hex16(r.res64))), field("res2048", str(r.res2048)),
+         field("residue-type", "1"), field("errors", object({field("gerbicz", std::to_string(r.nErrors))})),
+         field("fft-length", std::to_string(r.fftLength))};
+    if (r.proofPower > 0) {
+      f.push_back(field("proof", object({field("version", "1"), field("power", std::to_string(r.proofPower)),
+                                         field("hashsize", "64"), field("md5", str(r.proofMd5))})));
+    }
   } else {
-    f.push_back(field("res64", str(hex16(r.res64))));
-    f.push_back(field("fft-length", std::to_string(r.fftLength)));
-    f.push_back(field("shift-count", "0"));
-    f.push_back(field("error-code", str("00000000")));
+    f = {field("status", str(r.isPrime ? "P" : "C")), field("exponent", std::to_string(r.exponent)),
+         field("worktype", str("LL")), field("res64", str(hex16(r.res64))),
+         field("fft-length", std::to_string(r.fftLength)), field("shift-count", "0"),
+         field("error-code", str("00000000"))};
   }
   utsname u{};
   uname(&u);

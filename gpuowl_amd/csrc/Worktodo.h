@@ -38,10 +38,17 @@ struct ResultInfo {
   std::string res2048;       // PRP only
   u32 fftLength = 0;
   u32 nErrors = 0;           // PRP: Gerbicz errors
+  u32 proofPower = 0;        // PRP: power of the proof made by the run, 0 = none
+  std::string proofMd5;      // PRP: MD5 of that proof file (32 hex digits)
+  std::string sha3Hash;      // CERT: SHA3-256 of the final residue, 64 hex digits
+  u32 squarings = 0;         // CERT
   std::string aid, user, uid;
   std::string programName = "gpuowl_amd", programVersion = "1.0";
 };
 // One results line with the reference's keys and order: status, exponent, worktype, the
-// kind's fields, program {name, version, port, os {os, version, architecture}}, user, aid,
-// uid (empty ones omitted) and a UTC timestamp.
+// kind's fields (PRP: with the proof object {version 1, power, hashsize 64, md5} when the run
+// made a proof), program {name, version, port, os {os, version, architecture}}, user, aid,
+// uid (empty ones omitted) and a UTC timestamp. A CERT line has no status and starts with
+// worktype "Cert", exponent, sha3-hash, squarings, fft-length, shift-count, error-code
+// (src/Task.cpp:182-197).
 std::string resultJson(const ResultInfo& r);

@@ -4,10 +4,13 @@
 #include "Gpu.h"
 #include "GpuBatch.h"
 #include "Server.h"
+#include "Trig.h"
 #include "Tune.h"
 #include "Worktodo.h"
 
 #include <deque>
+#include <memory>
+#include <array>
 
 #include <cstdio>
 #include <cstring>
@@ -371,6 +374,33 @@ int owl_result_json(int kind, uint32_t E, int isPrime, uint64_t r64, const char*
   });
 }
 
+int owl_result_json_ex(const owl_result_t* in, char* out, size_t outLen) {
+  return guarded([&] {
+    if (!in) { throw std::invalid_argument("result"); }
+    auto sv = [](const char* p) { return std::string(p ? p : ""); };
+    ResultInfo r;
+    r.kind = WorkTask::Kind(in->kind);
+    r.exponent = in->exponent;
+    r.isPrime = in->isPrime != 0;
+    r.res64 = in->res64;
+    r.res2048 = sv(in->res2048);
+    r.fftLength = in->fftLength;
+    r.nErrors = in->nErrors;
+    r.proofPower = in->proofPower;
+    r.proofMd5 = sv(in->proofMd5);
+    r.sha3Hash = sv(in->sha3Hash);
+    r.squarings = in->squarings;
+    r.aid = sv(in->aid);
+    r.user = sv(in->user);
+    r.uid = sv(in->uid);
+    if (in->programName) { r.programName = in->programName; }
+    if (in->programVersion) { r.programVersion = in->programVersion; }
+    const std::string s = resultJson(r);
+    if (!out || outLen <= s.size()) { throw std::invalid_argument("result buffer too small"); }
+    snprintf(out, outLen, "%s", s.c_str());
+  });
+}
+
 int owl_proof_points(uint32_t E, uint32_t power, uint32_t* points, uint32_t cap, uint32_t* n) {
   return guarded([&] {
     std::vector<u32> p = proofPoints(E, power);
@@ -409,6 +439,37 @@ int owl_gpu_proof_verify(owl_gpu g, const char* proofPath, int* ok, int* isPrime
   });
 }
 
+int owl_gpu_cert(owl_gpu g, const char* certPath, uint32_t squarings, int removeFile, char* sha3Hex, size_t len,
+                 uint64_t* r64) {
+  return guarded([&] {
+    if (!certPath || !sha3Hex || len < 65) { throw std::invalid_argument("cert: path or hash buffer"); }
+    const u32 E = g->gpu->E;
+    const size_t nBytes = (E - 1) / 8 + 1;
+    Words start(nWords(E), 0);
+    {
+      std::unique_ptr<FILE, int (*)(FILE*)> f{fopen(certPath, "rb"), fclose};
+      if (!f) { throw std::invalid_argument(std::string("cannot read CERT start file ") + certPath); }
+      if (fread(start.data(), 1, nBytes, f.get()) != nBytes) {
+        throw std::invalid_argument(std::string("short CERT start file ") + certPath);
+      }
+    }
+    Words x = g->gpu->isCERT(start, squarings);
+    // the reference prints the digest's four little-endian u64 from the last to the first
+    const std::array<u64, 4> h = proofHash(E, x);
+    snprintf(sha3Hex, len, "%016llx%016llx%016llx%016llx", (unsigned long long)h[3], (unsigned long long)h[2],
+             (unsigned long long)h[1], (unsigned long long)h[0]);
+    if (r64) { *r64 = res64(x); }
+    if (removeFile) { std::remove(certPath); }
+  });
+}
+
+int owl_proof_md5(const char* path, char* md5Hex32, size_t len) {
+  return guarded([&] {
+    if (!path || !md5Hex32 || len < 33) { throw std::invalid_argument("md5: path or buffer"); }
+    snprintf(md5Hex32, len, "%s", md5File(path).c_str());
+  });
+}
+
 int owl_gpu_exp_mul(owl_gpu g, const uint32_t* A, uint64_t h, const uint32_t* B, int squareB, uint32_t* out,
                     uint32_t n) {
   return guarded([&] {
@@ -424,6 +485,15 @@ int owl_sha3_256(const void* data, size_t size, uint8_t* out) {
     if ((!data && size) || !out) { throw std::invalid_argument("sha3 arguments"); }
     std::array<u64, 4> d = Sha3_256{}.update(data, size).finish();
     std::memcpy(out, d.data(), 32);
+  });
+}
+
+int owl_trig_root1(uint32_t n, uint64_t k, double* c, double* s) {
+  return guarded([&] {
+    if (!n) { throw std::invalid_argument("root1: n = 0"); }
+    auto [cc, ss] = root1(n, k);
+    *c = cc;
+    *s = ss;
   });
 }
 
@@ -609,6 +679,10 @@ int owl_server_destroy(owl_server s) {
 
 int owl_server_submit(owl_server s, uint32_t E, int ll, uint32_t iters, uint32_t* id) {
   return guarded([&] { *id = s->server->submit(E, ll != 0, iters); });
+}
+
+int owl_server_arm_fault(owl_server s, uint32_t id, uint32_t k) {
+  return guarded([&] { s->server->armFault(id, k); });
 }
 
 int owl_server_step(owl_server s, uint32_t* nDone) {

@@ -3,7 +3,7 @@ src/Gpu.h:213-300, src/FFTConfig.h:69-87) over the C-ABI in include/owl.h."""
 import array
 import ctypes as C
 
-from ._lib import GpuInfo, LLResult, OwlError, PrpResult, ServeInfo, ServeResult, TuneEntry, check, load
+from ._lib import GpuInfo, LLResult, OwlError, PrpResult, ResultRec, ServeInfo, ServeResult, TuneEntry, check, load
 
 DATA, AUX, CHECK, BASE = 0, 1, 2, 3
 
@@ -167,6 +167,15 @@ class Gpu:
         check(self.lib.owl_gpu_proof_verify(self.h, str(proof_path).encode(), C.byref(ok), C.byref(prime)))
         return bool(ok.value), bool(prime.value)
 
+    def cert(self, cert_path, squarings, remove_file=False):
+        """(sha3-hash as the results line reports it, res64) of `squarings` squarings of the start
+        value in cert_path (the reference's M<E>.cert file)."""
+        buf = C.create_string_buffer(80)
+        r = C.c_uint64()
+        check(self.lib.owl_gpu_cert(self.h, str(cert_path).encode(), squarings, int(remove_file), buf, 80,
+                                    C.byref(r)))
+        return buf.value.decode(), r.value
+
     def exp_mul(self, A, h, B, square_b=False):
         """A^h * B (B squared first if square_b) as words."""
         n = n_words(self.E)
@@ -320,6 +329,11 @@ class Server:
         rid = C.c_uint32()
         check(self.lib.owl_server_submit(self.h, E, 1 if ll else 0, iters, C.byref(rid)))
         return rid.value
+
+    def arm_fault(self, rid, k):
+        """Failure-handling tests: flip a data bit of waiting PRP request rid after the round
+        that passes iteration k."""
+        check(self.lib.owl_server_arm_fault(self.h, rid, k))
 
     def step(self):
         n = C.c_uint32()
@@ -537,14 +551,24 @@ def prp_type1(words, E):
     return r.value, buf.value.decode()
 
 
-def result_json(kind, E, is_prime, res64, fft_length, res2048="", n_errors=0, aid="", user="", uid="",
-                program_name=None, program_version=None):
-    """One results line in the reference's JSON layout; kind "prp" or "ll"."""
-    k = {"prp": 1, "ll": 2}[kind]
-    buf = C.create_string_buffer(4096)
+def result_json(kind, E, is_prime=False, res64=0, fft_length=0, res2048="", n_errors=0, aid="", user="", uid="",
+                program_name=None, program_version=None, proof_power=0, proof_md5="", sha3_hash="", squarings=0):
+    """One results line in the reference's JSON layout; kind "prp", "ll" or "cert" (a PRP line
+    carries the proof object when proof_power > 0; a CERT line its sha3_hash and squarings)."""
     enc = lambda s: s.encode() if s is not None else None  # noqa: E731
-    check(load().owl_result_json(k, E, int(bool(is_prime)), res64, enc(res2048), fft_length, n_errors, enc(aid),
-                                 enc(user), enc(uid), enc(program_name), enc(program_version), buf, 4096))
+    r = ResultRec(kind={"prp": 1, "ll": 2, "cert": 3}[kind], exponent=E, isPrime=int(bool(is_prime)), res64=res64,
+                  res2048=enc(res2048), fftLength=fft_length, nErrors=n_errors, proofPower=proof_power,
+                  proofMd5=enc(proof_md5), sha3Hash=enc(sha3_hash), squarings=squarings, aid=enc(aid),
+                  user=enc(user), uid=enc(uid), programName=enc(program_name), programVersion=enc(program_version))
+    buf = C.create_string_buffer(4096)
+    check(load().owl_result_json_ex(C.byref(r), buf, 4096))
+    return buf.value.decode()
+
+
+def proof_md5(path):
+    """MD5 of a file as 32 lowercase hex digits (the proof object's md5 in a PRP results line)."""
+    buf = C.create_string_buffer(40)
+    check(load().owl_proof_md5(str(path).encode(), buf, 40))
     return buf.value.decode()
 
 
@@ -557,6 +581,13 @@ def proof_points(E, power):
     buf = (C.c_uint32 * n.value)()
     check(lib.owl_proof_points(E, power, buf, n.value, C.byref(n)))
     return list(buf)
+
+
+def root1(n, k):
+    """The twiddle table entry exp(-2 pi i k / n) as (cos, sin) doubles (TrigBufCache's root1)."""
+    c, s = C.c_double(), C.c_double()
+    check(load().owl_trig_root1(n, k, C.byref(c), C.byref(s)))
+    return c.value, s.value
 
 
 def sha3_256(data):

@@ -153,6 +153,37 @@ int owl_result_json(int kind, uint32_t E, int isPrime, uint64_t res64, const cha
                     uint32_t nErrors, const char* aid, const char* user, const char* uid, const char* programName,
                     const char* programVersion, char* out, size_t outLen);
 
+/* Results lines of every kind (Task::writeResultPRP / writeResultLL / writeResultCERT,
+ * src/Task.cpp:148-197): kind OWL_TASK_PRP, OWL_TASK_LL or OWL_TASK_CERT. PRP adds the
+ * "proof" object {version 1, power, hashsize 64, md5} after fft-length when proofPower > 0
+ * (proofMd5: owl_proof_md5 of the proof file). A CERT line has no status and carries worktype
+ * "Cert", exponent, sha3-hash (64 hex digits, as owl_gpu_cert writes it), squarings,
+ * fft-length, shift-count 0 and error-code. NULL strings count as "". */
+typedef struct {
+  int kind;
+  uint32_t exponent;
+  int isPrime;
+  uint64_t res64;
+  const char* res2048;
+  uint32_t fftLength, nErrors;
+  uint32_t proofPower;
+  const char* proofMd5;
+  const char* sha3Hash;
+  uint32_t squarings;
+  const char *aid, *user, *uid, *programName, *programVersion;
+} owl_result_t;
+int owl_result_json_ex(const owl_result_t* result, char* out, size_t outLen);
+/* MD5 of a file as 32 lowercase hex digits (proof::fileHash, src/Proof.cpp:32-39); len >= 33. */
+int owl_proof_md5(const char* path, char* md5Hex32, size_t len);
+
+/* Gpu::isCERT (src/Gpu.cpp:1665-1725): reads the start value from certPath ((E-1)/8+1
+ * little-endian bytes, the reference's M<E>.cert), squares it `squarings` times, and writes
+ * the SHA3-256 of the (E-1)/8+1 bytes of the result as the reference's results line reports
+ * it (its four little-endian 64-bit words from the last to the first, 64 hex digits; len >= 65)
+ * and the result's res64. removeFile deletes certPath when done, as the reference does. */
+int owl_gpu_cert(owl_gpu gpu, const char* certPath, uint32_t squarings, int removeFile, char* sha3Hex, size_t len,
+                 uint64_t* res64);
+
 /* PRP proofs (Proof / ProofSet, src/Proof.cpp:20-297; Gpu::expMul / expExp2, src/Gpu.cpp:813-889).
  * owl_gpu_prp_proof runs the full PRP-3 test (E iterations, Gerbicz checks as owl_gpu_prp) keeping the
  * residues of the 2^power proof points (owl_proof_points) under residueDir, builds the proof and
@@ -174,6 +205,11 @@ int owl_gpu_exp_mul(owl_gpu gpu, const uint32_t* A, uint64_t h, const uint32_t* 
                     uint32_t nWords);
 /* SHA3-256 (FIPS 202) of size bytes, the proof's hash (src/sha3.cpp): 32 bytes to out. */
 int owl_sha3_256(const void* data, size_t size, uint8_t* out);
+
+/* root1 (src/TrigBufCache.cpp:22-52): the table entry exp(-2 pi i k / n) = (*c, *s) that every
+ * twiddle table of the engine holds, computed in long double from an exactly reduced angle and
+ * rounded once to double. */
+int owl_trig_root1(uint32_t n, uint64_t k, double* c, double* s);
 
 /* crc32 (src/common.cpp:26-39): CRC-32/IEEE of size bytes. */
 int owl_crc32(const void* data, size_t size, uint32_t* crc);
@@ -241,6 +277,10 @@ int owl_server_create(uint64_t memBudget, uint32_t maxJobs, uint32_t blockSize, 
 int owl_server_destroy(owl_server server);
 int owl_server_submit(owl_server server, uint32_t E, int ll, uint32_t iters, uint32_t* id);
 int owl_server_step(owl_server server, uint32_t* nDone);
+/* Failure-handling tests (as owl_gpu_arm_fault): request id, a PRP request still waiting, gets one
+ * data bit flipped at the end of the round that passes iteration k; its next Gerbicz check fails
+ * and the job reloads its last verified state (src/Gpu.cpp:1561-1575). */
+int owl_server_arm_fault(owl_server server, uint32_t id, uint32_t k);
 /* up to cap finished results (oldest first); *n is how many were written */
 int owl_server_take(owl_server server, owl_serve_result_t* out, uint32_t cap, uint32_t* n);
 int owl_server_info(owl_server server, owl_serve_info_t* info);

@@ -8,7 +8,9 @@ Fixtures are named by exponent and kind:
   p4000037_prp            100,000 PRP-3 iterations of p = 4,000,037 (smallest prime >= 4,000,000)
   p139000003_{prp,ll}     20,000 PRP-3 / LL iterations of p = 139,000,003, and *_short: 200
   p<E>_prp                1,000 PRP-3 iterations of each of the first 64 primes >= 20,000,000
-Usage: python tests/golden/make_goldens.py [--jobs N] [names...]"""
+                          and of each of the 200 uneven exponents (splitmix64 seed 1234,
+                          log-uniform primes in [1e6, 6e7]; gpuowl_amd.workloads.uneven200)
+Usage: python tests/golden/make_goldens.py [--jobs N] [--set all|uneven200] [names...]"""
 import argparse, os, subprocess, sys, concurrent.futures as cf
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -66,7 +68,15 @@ def jobs():
     J[f"p{B}_ll_short"] = (B, 1, 200, 100)
     for p in primes_from(20_000_000, 64):
         J[f"p{p}_prp"] = (p, 0, 1000, 500)
+    for p in uneven200():
+        J[f"p{p}_prp"] = (p, 0, 1000, 500)
     return J
+
+
+def uneven200():
+    sys.path.insert(0, ROOT)
+    from gpuowl_amd import workloads
+    return workloads.uneven200()
 
 
 def run(name, E, ll, iters, step):
@@ -91,10 +101,21 @@ def run(name, E, ll, iters, step):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--set", default="all", choices=["all", "uneven200"])
+    ap.add_argument("--missing", action="store_true", help="only fixtures not written yet")
     ap.add_argument("names", nargs="*")
     a = ap.parse_args()
     all_jobs = jobs()
-    names = a.names or list(all_jobs)
+    if a.names:
+        names = a.names
+    elif a.set == "uneven200":
+        names = [f"p{p}_prp" for p in uneven200()]
+    else:
+        names = list(all_jobs)
+    if a.missing:
+        names = [n for n in names if not os.path.exists(os.path.join(HERE, f"res_{n}.txt"))]
+    # largest exponents first, so the longest jobs do not start last
+    names.sort(key=lambda n: -all_jobs[n][0] * all_jobs[n][2])
     with cf.ThreadPoolExecutor(max_workers=a.jobs) as ex:
         futs = [ex.submit(run, n, *all_jobs[n]) for n in names]
         for f in cf.as_completed(futs):

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 STRUCTS = {"owl_gpu_info_t": _lib.GpuInfo, "owl_prp_result_t": _lib.PrpResult, "owl_ll_result_t": _lib.LLResult,
            "owl_serve_result_t": _lib.ServeResult, "owl_serve_info_t": _lib.ServeInfo,
-           "owl_tune_entry_t": _lib.TuneEntry}
+           "owl_tune_entry_t": _lib.TuneEntry, "owl_result_t": _lib.ResultRec}
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs a C compiler")

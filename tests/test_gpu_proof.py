@@ -61,3 +61,22 @@ def test_proof_of_a_mersenne_prime(tmp_path):
 def test_proof_of_a_composite(tmp_path):
     announce("proof M1000003 power 5")
     proof_case(tmp_path, 1_000_003, 5, False)
+
+
+@pytest.mark.slow
+def test_engine_proof_passes_reference_algorithm(tmp_path):
+    # the engine's proof file, checked by the reference's algorithm restated in Python
+    # (tests/proofref.py, GMP arithmetic) rather than by the engine's own verifier
+    import oracle
+    from tests import proofref
+    E, power = 1_000_003, 9
+    announce("proof M1000003 power 9, reference-algorithm check")
+    path = tmp_path / f"{E}-{power}.proof"
+    with owl.Gpu(E) as g:
+        out = g.prp_proof(tmp_path / "residues", path, power, block_size=1000)
+    assert out.k == E and not out.isPrime and out.nErrors == 0
+    raw = path.read_bytes()
+    assert proofref.parse(raw)[:2] == (E, power)
+    assert proofref.points(E, power) == owl.proof_points(E, power)
+    # (altered files are rejected by the same verifier in tests/test_proof_cpu.py)
+    assert proofref.verify(raw, proofref.GmpArith(E, oracle)) == (True, False)

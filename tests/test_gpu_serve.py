@@ -118,3 +118,22 @@ def test_full_mersenne_prime_runs_to_completion():
     ll = next(r for r in res if r["ll"])
     assert ll["res64Type1"] == 0 and ll["res2048"] == ""
     assert json.loads(owl.result_json("ll", E, ll["isPrime"], ll["res64"], n))["status"] == "P"
+
+
+def test_fault_in_one_request_is_caught_and_recovered():
+    # one data bit of the first PRP request flips after the round that passes iteration 500;
+    # its Gerbicz check at 800 fails, it reloads the state verified at 400 and ends on the
+    # right residue; the other member of its group and the LL request are unaffected
+    reqs = [(2_000_003, "prp", 1600), (2_000_029, "prp", 1600), (2_000_081, "ll", 900)]
+    announce("serve fault", reqs)
+    with owl.Server(block_size=200, check_step=400) as s:
+        ids = {s.submit(E, k, n): (E, k, n) for E, k, n in reqs}
+        faulty = min(ids)
+        s.arm_fault(faulty, 500)
+        res = {r["id"]: r for r in s.drain()}
+    assert set(res) == set(ids)
+    for rid, (E, k, n) in ids.items():
+        r = res[rid]
+        assert r["status"] == 0, r
+        assert r["nErrors"] == (1 if rid == faulty else 0), r
+        assert r["res64"] == expected(E, k, n), (E, k, n)

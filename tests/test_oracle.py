@@ -50,6 +50,21 @@ def test_gmp_mulmod():
             assert got == a * b * (3 if mul3 else 1) % M
 
 
+def test_gmp_square_n():
+    # a^(2^n) mod 2^E - 1 against Python's pow, including n = 0, the residue M - 1 (= -1),
+    # and the non-canonical input M (= 0); a dropped or extra squaring changes the result
+    rnd = random.Random(11)
+    for E in (61, 127, 1279, 86243):
+        M = (1 << E) - 1
+        nw = oracle.n_words(E)
+        for a in (rnd.randrange(M), M - 1, 3, M):
+            wa = [(a >> (32 * i)) & 0xFFFFFFFF for i in range(nw)]
+            for n in (0, 1, 2, 7, 33):
+                got = sum(w << (32 * i) for i, w in enumerate(oracle.gmp_square_n(E, wa, n)))
+                assert got % M == pow(a, 1 << n, M), (E, a, n)
+                assert got < (1 << E)
+
+
 def test_bitlen_closed_form():
     # bits_k = ceil((k+1)E/N) - ceil(kE/N), extra(k)/N = ceil(kE/N) - kE/N; sum of bits = E
     for E, N in ((127, 16), (4423, 256), (100003, 4096), (4000037, 262144)):

@@ -49,3 +49,34 @@ def test_bad_power_is_an_error():
             owl.proof_points(1_000_003, power)
     with pytest.raises(owl.OwlError):
         owl.proof_points(1_000_004, 4)   # E must be odd
+
+
+def test_reference_proof_algorithm_round_trip():
+    # tests/proofref.py restates the reference's proof construction and verification; a proof it
+    # builds for a Mersenne prime and for a composite verifies, and one altered bit of B or of any
+    # middle makes it invalid (each enters the hash chain and the final identity)
+    from tests import proofref
+    for E, power, prime in ((4423, 3, True), (4099, 4, False)):
+        raw = proofref.make(E, power)
+        assert proofref.verify(raw) == (True, prime)
+        assert proofref.points(E, power) == owl.proof_points(E, power)
+        _, _, b, mids = proofref.parse(raw)
+        head = len(raw) - (power + 1) * proofref.nbytes(E)
+        for part in range(power + 1):
+            bad = bytearray(raw)
+            bad[head + part * proofref.nbytes(E) + 3] ^= 0x10
+            assert proofref.verify(bytes(bad))[0] is False, (E, part)
+
+
+def test_reference_proof_gmp_arithmetic_agrees():
+    # the GMP-backed arithmetic the GPU proof test uses for E near 10^6 equals the Python one
+    import oracle
+    from tests import proofref
+    rnd = random.Random(9)
+    E = 4423
+    py, gmp = proofref.Arith(E), proofref.GmpArith(E, oracle)
+    for _ in range(5):
+        a, b, h = rnd.randrange(py.M), rnd.randrange(py.M), rnd.getrandbits(64)
+        assert gmp.mul(a, b) == py.mul(a, b) == a * b % py.M
+        assert gmp.pow(a, h) == py.pow(a, h) == pow(a, h, py.M)
+        assert gmp.square_n(a, 13) == py.square_n(a, 13) == pow(a, 2**13, py.M)

@@ -5,6 +5,8 @@ the reference's JSON layout (src/Task.cpp:86-170)."""
 import json
 import random
 
+import pytest
+
 import gpuowl_amd as owl
 
 AID = "FEEE9DCD59A0855711265C1165C4C693"
@@ -75,3 +77,49 @@ def test_result_lines():
     assert ll["program"]["name"] == "prpll"
     assert json.loads(owl.result_json("ll", 57885161, False, 1, 3145728, user='a "quoted" name'))["user"] == \
         'a "quoted" name'
+
+
+def test_result_line_with_proof_object(tmp_path):
+    # a PRP run that made a proof: {"version":1, "power":p, "hashsize":64, "md5":<file md5>} after
+    # fft-length (src/Task.cpp:151-163)
+    import hashlib
+    proof = tmp_path / "x.proof"
+    proof.write_bytes(b"PRP PROOF\nVERSION=2\n" + bytes(range(256)) * 300)
+    md5 = owl.proof_md5(proof)
+    assert md5 == hashlib.md5(proof.read_bytes()).hexdigest()
+    prp = json.loads(owl.result_json("prp", 124647911, False, 5, 7340032, res2048="0" * 512, proof_power=8,
+                                     proof_md5=md5))
+    assert list(prp)[:9] == ["status", "exponent", "worktype", "res64", "res2048", "residue-type", "errors",
+                             "fft-length", "proof"]
+    assert prp["proof"] == {"version": 1, "power": 8, "hashsize": 64, "md5": md5}
+    # no proof, no object
+    assert "proof" not in json.loads(owl.result_json("prp", 124647911, False, 5, 7340032, res2048="0" * 512))
+
+
+def test_md5_matches_hashlib(tmp_path):
+    # RFC 1321 test suite plus the 55/56/64-byte padding boundaries
+    for data in (b"", b"a", b"abc", b"message digest", b"abcdefghijklmnopqrstuvwxyz", b"x" * 55, b"y" * 56,
+                 b"z" * 64, b"q" * 119, bytes(range(256)) * 517):
+        f = tmp_path / "m"
+        f.write_bytes(data)
+        assert owl.proof_md5(f) == hashlib_md5(data), len(data)
+    f.write_bytes(b"abc")
+    assert owl.proof_md5(f) == "900150983cd24fb0d6963f7d28e17f72"
+
+
+def hashlib_md5(data):
+    import hashlib
+    return hashlib.md5(data).hexdigest()
+
+
+def test_cert_result_line():
+    # Task::writeResultCERT (src/Task.cpp:182-197): no status; worktype, exponent, sha3-hash,
+    # squarings, fft-length, shift-count, error-code, then the program block and the rest
+    h = "0123456789abcdef" * 4
+    cert = json.loads(owl.result_json("cert", 120000023, fft_length=6815744, sha3_hash=h, squarings=12345, aid=AID))
+    assert list(cert) == ["worktype", "exponent", "sha3-hash", "squarings", "fft-length", "shift-count",
+                          "error-code", "program", "aid", "timestamp"]
+    assert (cert["worktype"], cert["exponent"], cert["sha3-hash"], cert["squarings"], cert["fft-length"],
+            cert["shift-count"], cert["error-code"]) == ("Cert", 120000023, h, 12345, 6815744, 0, "00000000")
+    with pytest.raises(owl.OwlError):
+        owl.result_json("cert", 120000023, fft_length=6815744, sha3_hash="abc", squarings=1)
