@@ -52,6 +52,42 @@ def cpu_baseline(E, iters):
             "secs": rec["secs"]}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_batched_baseline(owl, exps, iters, procs):
+    """The GMP oracle as a batched CPU baseline: `procs` single-threaded processes at once, each
+    squaring a full-width residue of one of the exponents `iters` times; aggregate
+    squarings*words/s with each exponent counted at its FFT length N (the GPU's unit)."""
+    import oracle
+    cli = oracle.gmp_cli()
+    if not os.path.exists(cli):
+        return None
+    chosen = [exps[i % len(exps)] for i in range(procs)]
+    t0 = time.time()
+    ps = [subprocess.Popen([cli, str(E), str(iters), "0", "0", "1"], stdout=subprocess.PIPE, text=True)
+          for E in chosen]
+    secs = []
+    for p in ps:
+        out, _ = p.communicate(timeout=600)
+        secs.append(json.loads(out.strip().splitlines()[-1])["secs"])
+    wall = time.time() - t0
+    words = sum(owl.best_fit(E, "1K:3:256")[1] for E in chosen)
+    return {"value": round(words * iters / max(secs)), "unit": "squarings*words/s", "cores": procs,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{procs} GMP processes at once (oracle/gmp_loop.c, one thread each), {iters} squarings of a "
+                      f"full-width residue mod 2^p-1 each, p = the first {len(set(chosen))} primes >= 20,000,000, "
+                      f"each counted at N = 1,572,864 words",
+            "wall_s": round(wall, 2)}
+
+
 def engine_key(info):
     return (info["spec"], info["R"], info["long_carry"], info["row"], info["col"])
 
@@ -215,11 +251,14 @@ def main():
     ap.add_argument("--batch-iters", type=int, default=100)
     ap.add_argument("--no-batch-tune", action="store_true", help="batched legs with the single-job splits")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sample")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-lambda", type=float, default=4.0)
-    ap.add_argument("--serve-slice", type=int, default=20_000)
-    ap.add_argument("--serve-full-below", type=int, default=0,
-                    help="requests below this exponent run to completion (the full serving config uses 10,000,000)")
+    ap.add_argument("--serve-requests", type=int, default=8,
+                    help="the first requests of the serving workload (the default 8 hold one run to completion)")
+    ap.add_argument("--serve-lambda", type=float, default=1.0)
+    ap.add_argument("--serve-slice", type=int, default=50_000)
+    ap.add_argument("--serve-full-below", type=int, default=10_000_000,
+                    help="requests below this exponent run to completion, the rest one slice")
+    ap.add_argument("--cpu-batch-procs", type=int, default=0,
+                    help="GMP processes of the batched CPU baseline (0 = the host's cores, at most 16)")
     ap.add_argument("--profile", action="store_true",
                     help="only the profiled workload: K steady iterations and one state copy, no JSON line")
     ap.add_argument("--profile-batch", action="store_true",
@@ -327,10 +366,15 @@ def main():
                      a.serve_full_below)
         note("serving sample done")
 
-    cpu = None
+    cpu = cpu_batch = None
     if not a.no_cpu:
         cpu = cpu_baseline(E, a.cpu_iters)
+        if cpu:
+            cpu["cpu_model"] = cpu_model()
         note("CPU baseline done")
+        procs = a.cpu_batch_procs or min(os.cpu_count() or 1, 16)
+        cpu_batch = leg(cpu_batched_baseline, owl, workloads.uniform64_20m(), 64, procs)
+        note("batched CPU baseline done")
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "iter/s", "n_gpus": 1, "steps": a.steps,
             "warmup": a.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
@@ -340,7 +384,9 @@ def main():
             "roofline": roofline, "cpu_baseline": cpu,
             "extra": {"us_per_iter": round(us, 2), "us_per_iter_graph": round(us_graph, 2),
                       "us_per_iter_prp_with_gerbicz": round(us_prp, 2), "roe": roe, "parity": parity,
-                      "batched": batched, "served": served}}
+                      "batched": batched, "served": served, "cpu_baseline_batched": cpu_batch,
+                      "reference_opencl": "not measured: no OpenCL CPU runtime (PoCL) on the box and the "
+                                          "reference is not shipped there"}}
     print(json.dumps(line))
 
 

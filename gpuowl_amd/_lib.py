@@ -106,6 +106,7 @@ SIGNATURES = {
     "owl_gpu_is_equal": [vp, i32, i32, P(i32)],
     "owl_gpu_copy": [vp, i32, i32],
     "owl_gpu_read_roe": [vp, P(u32), P(dbl), P(dbl), P(dbl)],
+    "owl_gpu_measure_roe": [vp, u32, u32, P(u32), P(dbl), P(dbl), P(dbl), P(dbl)],
     "owl_gpu_prp": [vp, u32, u32, u32, P(u64), u32, u32, P(PrpResult)],
     "owl_gpu_arm_fault": [vp, u32],
     "owl_gpu_ll": [vp, u32, P(u64), u32, u32, P(LLResult)],

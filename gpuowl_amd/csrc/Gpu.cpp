@@ -47,6 +47,7 @@ bool equals9(const Words& w) {
 Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fftIn} {
   GpuOptions opt = optIn;
   owlCheck(owl_hip_set_device(opt.device), "set device");
+  device = opt.device;
   W = fft.shape.width;
   H = fft.shape.middle * fft.shape.height;
   N = fft.size();
@@ -817,6 +818,18 @@ Words Gpu::isCERT(const Words& start, u32 squarings) {
   squareLoop(DATA, 0, squarings);
   Words r = readData(DATA);
   r.resize(nWords(E), 0);
+  return r;
+}
+
+RoeInfo Gpu::measureROE(u32 iters, u32 warmup) {
+  if (!iters || iters > OWL_ROE_SIZE) { throw std::invalid_argument("measureROE: 1 .. 100,000 iterations"); }
+  writeIn(DATA, makeWords(E, 3));
+  if (warmup) { squareLoop(DATA, 0, warmup); }
+  readROE();  // the warm-up iterations are not counted
+  restartRoeWindow(iters);
+  squareLoop(DATA, warmup, warmup + iters);
+  RoeInfo r = readROE();
+  restartRoeWindow();
   return r;
 }
 

@@ -76,6 +76,7 @@ public:
   // geometry
   const u32 E;
   const FFTConfig fft;
+  int device = 0;  // the HIP device the engine's queue and buffers are on
   u32 N, ND, W, H, R, nb, rowT, colT, rowLds, colLds;
   std::string rowKernel, colKernel;  // colKernel: what a launch of this job alone runs
   // Column pass with several column pairs per group ("colx<H>m", compiled for every k_colx
@@ -138,6 +139,9 @@ public:
   // (src/Gpu.cpp:738-765)
   void writeState(const Words& check, u32 blockSize);
   double timePRP(u32 iters = 1000, u32 warmup = 30, u32 blockSize = 200);
+  // Gpu::measureROE (src/Gpu.cpp:1268-1335): the round-off of `iters` squarings of a PRP-3 run
+  // after `warmup` unmeasured ones, one sample (the max over the words) per squaring.
+  RoeInfo measureROE(u32 iters = 2000, u32 warmup = 50);
   double timeSquares(u32 iters, u32 warmup, bool useGraph);
 
   // raw access for tests and the C-ABI

@@ -224,6 +224,21 @@ int owl_gpu_read_roe(owl_gpu g, uint32_t* n, double* maxRoe, double* mean, doubl
   });
 }
 
+int owl_gpu_measure_roe(owl_gpu g, uint32_t iters, uint32_t warmup, uint32_t* n, double* maxRoe, double* mean,
+                        double* sd, double* z) {
+  return guarded([&] {
+    RoeInfo r = g->gpu->measureROE(iters, warmup);
+    *n = r.N;
+    *maxRoe = r.max;
+    *mean = r.mean;
+    *sd = r.sd;
+    // Gumbel fit by moments (RoeInfo, src/Gpu.h:62-81): beta = sd*sqrt(6)/pi, mu = mean - gamma*beta,
+    // Z = (0.5 - mu) / beta
+    const double beta = r.sd * 0.779696801233676, mu = r.mean - beta * 0.577215664901533;
+    *z = (r.N && beta > 0) ? (0.5 - mu) / beta : 0.0;
+  });
+}
+
 int owl_gpu_prp(owl_gpu g, uint32_t iters, uint32_t blockSize, uint32_t checkStep, uint64_t* res, uint32_t resStep,
                 uint32_t resCap, owl_prp_result_t* out) {
   return guarded([&] {
@@ -848,9 +863,9 @@ int owl_gpu_pass(owl_gpu g, int pass, int arg) {
     switch (pass) {
       case OWL_PASS_LEADIN: G.rowLeadIn(buf(arg)); break;
       case OWL_PASS_SQUARE: G.colSquare(); break;
-      case OWL_PASS_FUSED: G.rowFused(arg != 0, OWL_ROE_SIZE); break;
+      case OWL_PASS_FUSED: G.rowFused(arg != 0, G.nextRoeSlot()); break;
       case OWL_PASS_CARRYOUT:
-        G.rowCarryOut(buf(arg), false, false, OWL_ROE_SIZE);
+        G.rowCarryOut(buf(arg), false, false, G.nextRoeSlot());
         G.carryAll(buf(arg));
         break;
       case OWL_PASS_SPECTRUM: G.colFwdSpec(); break;

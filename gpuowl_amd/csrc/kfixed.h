@@ -51,8 +51,14 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
     if (TOT % T == 0 || g < TOT) {
       const int f = g / NB, j = g % NB;
       const T2* x = lds + f * REG;
+      if constexpr (NB % 16 == 0) {  // padi(j + k*NB) = padi(j) + k*(NB + NB/16)
+        const T2* xj = x + padi(j);
 #pragma unroll
-      for (int k = 0; k < R; ++k) { v[m][k] = x[padi(j + k * NB)]; }
+        for (int k = 0; k < R; ++k) { v[m][k] = xj[k * (NB + NB / 16)]; }
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) { v[m][k] = x[padi(j + k * NB)]; }
+      }
       if (Ns > 1) {
         const int p = j % Ns;
 #pragma unroll
@@ -77,8 +83,18 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
       T2* x = lds + f * REG;
       const int p = j % Ns;
       const int base = (j - p) * R + p;
+      if constexpr (Ns % 16 == 0) {  // padi(base + k*Ns) = padi(base) + k*(Ns + Ns/16)
+        T2* xb = x + padi(base);
 #pragma unroll
-      for (int k = 0; k < R; ++k) { x[padi(base + k * Ns)] = v[m][k]; }
+        for (int k = 0; k < R; ++k) { xb[k * (Ns + Ns / 16)] = v[m][k]; }
+      } else if constexpr (Ns == 1 && 16 % R == 0) {  // base = j*R: the R outputs stay in one 16-run
+        T2* xb = x + padi(base);
+#pragma unroll
+        for (int k = 0; k < R; ++k) { xb[k] = v[m][k]; }
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) { x[padi(base + k * Ns)] = v[m][k]; }
+      }
     }
   }
   if (priv) {
@@ -206,23 +222,27 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
   const T2* other = (mode == COL_MUL) ? J.freq[oi] : nullptr;
   const size_t blockStride = (size_t)W * R;
   const int per = 2 * R;
+  // Element (line l, column h of pair q) lies at offset (l/R)*W*R + 2qR + hR + l%R, below ND < 2^32.
+  // The host runs this pass only when R divides H/16 = NB0, so for the lines l = j + k*NB0 of a
+  // stage-0 butterfly the offset is off(j) + k*kStride: one add per element.
+  const uint32_t kStride = (uint32_t)(NB0 >> lgR) * (uint32_t)(W * R);
+  auto offset0 = [&](int q, int h, int j) -> uint32_t {
+    return (uint32_t)(j >> lgR) * (uint32_t)(W * R) + (uint32_t)(2 * q * R + (h << lgR) + (j & (R - 1)));
+  };
 
   // Lane order within 2R lanes: (column h, line-in-block r), so 2R lanes read 2R*16
   // contiguous bytes of one line block.
   T2 xin[MB0][C0];
   auto loadPair = [&](int q, int tid) {
-    const size_t colBase = (size_t)2 * q * R;
 #pragma unroll
     for (int m = 0; m < MB0; ++m) {
       const int g = tid + m * T;
       if (NBT % T == 0 || g < NBT) {
         const int h = (g >> lgR) & 1;
         const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
+        const uint32_t o = offset0(q, h, j);
 #pragma unroll
-        for (int k = 0; k < C0; ++k) {
-          const int l = j + k * NB0;
-          xin[m][k] = ldx<NT>(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)));
-        }
+        for (int k = 0; k < C0; ++k) { xin[m][k] = ldx<NT>(io + (o + k * kStride)); }
       }
     }
   };
@@ -348,12 +368,9 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
           for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw(j * k)); }
           Dft<C0, true>::run(v);
           const T2 base = tw.mid(h, j);
+          const uint32_t o = offset0(q, h, j);
 #pragma unroll
-          for (int k = 0; k < C0; ++k) {
-            const int l = j + k * NB0;
-            stx<NT>(io + (size_t)(l >> lgR) * blockStride + colBase + (h << lgR) + (l & (R - 1)),
-                    cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base));
-          }
+          for (int k = 0; k < C0; ++k) { stx<NT>(io + (o + k * kStride), cmulc(v[k], k ? cmul(base, tw.s(h, k)) : base)); }
         }
       }
     }
@@ -505,6 +522,55 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
 }
 
 // ---------------------------------------------------------------- row pass
+// Weight data of one line read through the scalar cache: the line index is uniform across the
+// group (readfirstlane), so the loads are scalar and the values live in SGPRs.
+#define CONSTANT __attribute__((address_space(4)))
+struct LineWs {
+  double w0, w1;
+  uint32_t e0, e1;
+};
+__device__ __forceinline__ LineWs lineWs(const LineW* p) {
+  const CONSTANT LineW* q = (const CONSTANT LineW*)p;
+  return {q->w[0], q->w[1], q->e[0], q->e[1]};
+}
+// Bits of word (line, column, rep) and whether its weight exponent wrapped (see wordWeight).
+__device__ __forceinline__ int wordBits(const JobDesc& J, uint32_t le, uint32_t ce, bool& wrap) {
+  uint32_t e = le + ce;
+  wrap = e >= J.N;
+  uint32_t ex = wrap ? e - J.N : e;
+  return J.bitsLo + (ex < J.rE);
+}
+
+// Tagged stairway access through the global address space.
+__device__ __forceinline__ uint64_t stairLoad(const uint64_t* p) {
+  return __hip_atomic_load((const GLOBAL uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stairStore(uint64_t* p, uint64_t v) {
+  __hip_atomic_store((GLOBAL uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Polls N tagged carries until every tag is current. All N loads are in flight together and a
+// round that finds any of them stale reloads all N (a current tag stays current for the rest of
+// the launch), so the loop body has no per-carry branches. Bounded: a time-out is flagged to the
+// host and the kernel still exits.
+template<int N>
+__device__ __forceinline__ void pollAll(const JobDesc& J, uint64_t* const* p, uint32_t tag, uint64_t* v) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) { v[k] = stairLoad(p[k]); }
+  for (uint32_t n = 0;; ++n) {
+    bool ready = true;
+#pragma unroll
+    for (int k = 0; k < N; ++k) { ready &= (uint32_t)(v[k] >> 48) == tag; }
+    if (ready) { break; }
+    if (n >= OWL_SPIN_LIMIT) {
+      atomicOr(J.flags + 2, 1u);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int k = 0; k < N; ++k) { v[k] = stairLoad(p[k]); }
+  }
+}
+
 // Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 + 64
 // (+ ROWTW_SIZE*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle table.
 // Modes, options and the tagged stairway as k_row.
@@ -524,7 +590,8 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   const Tw tw{LDS_TW ? twTab : J.twRow};
   const int nb = J.nb;
   __shared__ int ticketSlot;
-  const int g = (mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x;
+  // the block index is uniform: line weights then come through the scalar cache
+  const int g = __builtin_amdgcn_readfirstlane((mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x);
   const int lb = (g == nb) ? 0 : g;
   const int tid = threadIdx.x;
   const bool ll = opts & 1, mul3 = opts & 2;
@@ -533,31 +600,23 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   T2* blk = J.freq[fi] + (size_t)lb * LW;
   if constexpr (LDS_TW) { fillRowTw<W, T>(J.twRow, twTab, tid); }  // read after the first barrier below
 
-  // Thread tid owns columns c = tid + k*T (k < F0) of the R lines of the block.
+  // Thread tid owns columns c = tid + k*T (k < F0) of the R lines of the block. v holds the
+  // inverse transform, then the balanced digits (as doubles), then the weighted digits.
   T2 v[R][F0];
-  int2 d[R][F0];
   ColW cw[F0];
 #pragma unroll
   for (int k = 0; k < F0; ++k) { cw[k] = gld(J.colW + tid + k * T); }
-  double wt[R][F0][2];  // forward weights of the thread's words
-  auto weights = [&]() {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const LineW lw = gld(J.lineW + lb * R + r);
-#pragma unroll
-      for (int k = 0; k < F0; ++k) {
-        wt[r][k][0] = wordWeight(J, lw, cw[k], 0).w;
-        wt[r][k][1] = wordWeight(J, lw, cw[k], 1).w;
-      }
-    }
-  };
+  const LineW* lwBase = J.lineW + lb * R;
 
   if (mode == ROW_LEADIN) {
     const int2* src = J.words[wi] + (size_t)lb * LW;
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) { d[r][k] = gld(src + (tid + k * T) * R + r); }
+      for (int r = 0; r < R; ++r) {
+        const int2 d = gld(src + (tid + k * T) * R + r);
+        v[r][k] = make_double2(d.x, d.y);
+      }
     }
   } else {
 #pragma unroll
@@ -579,42 +638,50 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       Dft<F0, true>::run(v[r]);
     }
 
-    // unweight, round, carry along the block's 2R words per column
-    float roe = 0;
-    int64_t s[F0];
+    // Unweight, round, carry along the block's 2R words per column. The carry chain runs in
+    // FP64: every value is an integer of magnitude below 2^51, so sums, scaling by powers of two
+    // and floor are exact. Word u of b bits splits into the carry floor(u/2^b + 1/2) and the
+    // balanced digit u - carry*2^b in [-2^(b-1), 2^(b-1)), as the integer chain would.
+    double roe = 0;
+    double cs[F0];
 #pragma unroll
-    for (int k = 0; k < F0; ++k) { s[k] = 0; }
+    for (int k = 0; k < F0; ++k) { cs[k] = 0; }
+    const double m3 = mul3 ? 3.0 : 1.0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int l = lb * R + r;
-      const LineW lw = gld(J.lineW + l);
+      const LineW lw = gld(lwBase + r);
 #pragma unroll
       for (int k = 0; k < F0; ++k) {
-        const int c = tid + k * T;
-        int32_t dd[2];
 #pragma unroll
         for (int rep = 0; rep < 2; ++rep) {
-          WordW ww = wordWeight(J, lw, cw[k], rep);
-          int64_t u = roundWord((rep ? v[r][k].y : v[r][k].x) * ww.iw, roe);
-          if (mul3) { u *= 3; }
-          if (ll && l == 0 && c == 0 && rep == 0) { u -= 2; }
-          u += s[k];
-          int64_t t = lowBits64(u, ww.bits);
-          s[k] = (u - t) >> ww.bits;
-          dd[rep] = (int32_t)t;
+          bool wrap;
+          const int b = wordBits(J, lw.e[rep], cw[k].e, wrap);
+          const double iw = __builtin_amdgcn_ldexp(lw.iw[rep] * cw[k].iw, wrap ? 1 : 0);
+          const double x = rep ? v[r][k].y : v[r][k].x;
+          const double t = fma(x, iw, OWL_RNDVAL) - OWL_RNDVAL;
+          roe = fmax(roe, fabs(fma(x, iw, -t)));
+          double u = fma(t, m3, cs[k]);
+          if (ll && r == 0 && k == 0 && rep == 0 && lb == 0 && tid == 0) { u -= 2; }
+          const double c = floor(__builtin_amdgcn_ldexp(u, -b) + 0.5);
+          const double dgt = u - __builtin_amdgcn_ldexp(c, b);
+          cs[k] = c;
+          if (rep) {
+            v[r][k].y = dgt;
+          } else {
+            v[r][k].x = dgt;
+          }
         }
-        d[r][k] = make_int2(dd[0], dd[1]);
       }
     }
     if (mode == ROW_CARRYOUT) {
-      roeReport(J, roeSlot, roe);
+      roeReport(J, roeSlot, (float)roe);
       int2* dst = J.words[wi] + (size_t)lb * LW;
 #pragma unroll
       for (int k = 0; k < F0; ++k) {
         const int c = tid + k * T;
-        gst(J.carry[0] + (size_t)lb * W + c, s[k]);
+        gst(J.carry[0] + (size_t)lb * W + c, (int64_t)cs[k]);
 #pragma unroll
-        for (int r = 0; r < R; ++r) { gst(dst + c * R + r, d[r][k]); }
+        for (int r = 0; r < R; ++r) { gst(dst + c * R + r, make_int2((int)v[r][k].x, (int)v[r][k].y)); }
       }
       return;
     }
@@ -622,55 +689,49 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
     if (g < nb) {
 #pragma unroll
       for (int k = 0; k < F0; ++k) {
-        __hip_atomic_store(J.stair + (size_t)g * W + tid + k * T, packCarry(s[k], tag), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        stairStore(J.stair + (size_t)g * W + tid + k * T, packCarry((int64_t)cs[k], tag));
       }
     }
     if (g == 0) {
-      roeReport(J, roeSlot, roe);
+      roeReport(J, roeSlot, (float)roe);
       return;
     }
-    roeWave(roeSlot, roe, roePart);
-    const int pred = g - 1;
+    roeWave(roeSlot, (float)roe, roePart);
     uint64_t* src[F0];
     uint64_t got[F0];
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
       const int c = tid + k * T;
-      src[k] = J.stair + (size_t)pred * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
+      src[k] = J.stair + (size_t)(g - 1) * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
     }
-    // the first poll loads go out now; the weights of the final weighting, which do not
-    // depend on the carry, are computed while they are in flight
-    pollStart<F0>(src, got);
-    weights();
-    pollFinish<F0>(J, src, tag, got);
+    pollAll<F0>(J, src, tag, got);
     if (g == nb) {  // group 0 has read block 0
 #pragma unroll
       for (int k = 0; k < F0; ++k) { src[k] = J.stair + tid + k * T; }
       uint64_t seen[F0];
-      pollCarries<F0>(J, src, tag, seen);
+      pollAll<F0>(J, src, tag, seen);
     }
+    // ripple the predecessor's carry through the column's 2R words until it dies
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
-      int64_t carry = unpackCarry(got[k]);
+      double carry = (double)unpackCarry(got[k]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if (carry) {
-          const LineW lw = gld(J.lineW + lb * R + r);
-          WordW w0 = wordWeight(J, lw, cw[k], 0);
-          int64_t t = (int64_t)d[r][k].x + carry;
-          int64_t nd = lowBits64(t, w0.bits);
-          carry = (t - nd) >> w0.bits;
-          d[r][k].x = (int32_t)nd;
+        if (carry != 0) {
+          const LineWs lw = lineWs(lwBase + r);
+          bool wrap;
+          const int b0 = wordBits(J, lw.e0, cw[k].e, wrap);
+          double t = v[r][k].x + carry;
+          carry = floor(__builtin_amdgcn_ldexp(t, -b0) + 0.5);
+          v[r][k].x = t - __builtin_amdgcn_ldexp(carry, b0);
           if (r == R - 1) {
-            d[r][k].y += (int32_t)carry;
+            v[r][k].y += carry;
             carry = 0;
-          } else if (carry) {
-            WordW w1 = wordWeight(J, lw, cw[k], 1);
-            t = (int64_t)d[r][k].y + carry;
-            nd = lowBits64(t, w1.bits);
-            carry = (t - nd) >> w1.bits;
-            d[r][k].y = (int32_t)nd;
+          } else if (carry != 0) {
+            const int b1 = wordBits(J, lw.e1, cw[k].e, wrap);
+            t = v[r][k].y + carry;
+            carry = floor(__builtin_amdgcn_ldexp(t, -b1) + 0.5);
+            v[r][k].y = t - __builtin_amdgcn_ldexp(carry, b1);
           }
         }
       }
@@ -678,11 +739,18 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   }
 
   // weight, then forward stage 0 (radix F0, Ns = 1) straight from registers
-  if (mode == ROW_LEADIN) { weights(); }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
+    const LineWs lw = lineWs(lwBase + r);
 #pragma unroll
-    for (int k = 0; k < F0; ++k) { v[r][k] = make_double2(d[r][k].x * wt[r][k][0], d[r][k].y * wt[r][k][1]); }
+    for (int k = 0; k < F0; ++k) {
+      bool wrap0, wrap1;
+      wordBits(J, lw.e0, cw[k].e, wrap0);
+      wordBits(J, lw.e1, cw[k].e, wrap1);
+      const double w0 = __builtin_amdgcn_ldexp(lw.w0 * cw[k].w, wrap0 ? -1 : 0);
+      const double w1 = __builtin_amdgcn_ldexp(lw.w1 * cw[k].w, wrap1 ? -1 : 0);
+      v[r][k] = make_double2(v[r][k].x * w0, v[r][k].y * w1);
+    }
     Dft<F0, false>::run(v[r]);
   }
   __syncthreads();  // every thread has finished reading the inverse's LDS

@@ -134,6 +134,13 @@ class Gpu:
         check(self.lib.owl_gpu_read_roe(self.h, C.byref(n), C.byref(mx), C.byref(mean), C.byref(sd)))
         return {"n": n.value, "max": mx.value, "mean": mean.value, "sd": sd.value}
 
+    def measure_roe(self, iters=2000, warmup=50):
+        """Round-off statistics of `iters` squarings (Gpu::measureROE) with the Gumbel Z."""
+        n, mx, mean, sd, z = C.c_uint32(), C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        check(self.lib.owl_gpu_measure_roe(self.h, iters, warmup, C.byref(n), C.byref(mx), C.byref(mean), C.byref(sd),
+                                           C.byref(z)))
+        return {"n": n.value, "max": mx.value, "mean": mean.value, "sd": sd.value, "z": z.value}
+
     # ---- workflows
     def arm_fault(self, k):
         """The next prp() flips one data bit right after iteration k (failure-handling tests)."""

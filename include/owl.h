@@ -94,6 +94,12 @@ int owl_gpu_copy(owl_gpu gpu, int dst, int src);
 /* Gpu::readROE (src/Gpu.cpp:624-637): statistics of the max round-off per iteration since the last read. */
 int owl_gpu_read_roe(owl_gpu gpu, uint32_t* n, double* maxRoe, double* mean, double* sd);
 
+/* Gpu::measureROE (src/Gpu.cpp:1268-1335) with RoeInfo's Gumbel fit (src/Gpu.h:62-81): round-off of
+ * iters (<= 100,000) squarings of a PRP-3 run from 3 after warmup unmeasured ones, one sample per
+ * squaring; *z = (0.5 - mu) / beta with beta = sd*sqrt(6)/pi, mu = mean - 0.5772*beta (the reference
+ * fits its bits-per-word limits at Z = 28, src/tune.cpp:106-172). */
+int owl_gpu_measure_roe(owl_gpu gpu, uint32_t iters, uint32_t warmup, uint32_t* n, double* maxRoe, double* mean,
+                        double* sd, double* z);
 /* Gpu::isPrimePRP (src/Gpu.cpp:1400-1584) without savefiles or proofs (owl_gpu_prp_saved and
  * owl_gpu_prp_proof below add them): iters PRP-3 iterations from x0 = 3 with a Gerbicz check every
  * checkStep (multiple of blockSize). A full test (iters == E) runs on to the next multiple of
@@ -326,6 +332,7 @@ int owl_tune_save(const char* path);
  *   OWL_PASS_SQUARE   state <- column pass with the squaring
  *   OWL_PASS_FUSED    state <- row pass with the fused carry (arg = 1: LL)
  *   OWL_PASS_CARRYOUT state -> words[arg], fully carried
+ *   (both measure their round-off into the next slot that owl_gpu_read_roe reports) 
  *   OWL_PASS_SPECTRUM spectrum buffer <- forward column transform of itself
  *   OWL_PASS_MUL      state <- column pass multiplying by the spectrum buffer */
 enum { OWL_PASS_LEADIN = 0, OWL_PASS_SQUARE = 1, OWL_PASS_FUSED = 2, OWL_PASS_CARRYOUT = 3, OWL_PASS_SPECTRUM = 4,

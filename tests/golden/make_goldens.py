@@ -10,6 +10,8 @@ Fixtures are named by exponent and kind:
   p<E>_prp                1,000 PRP-3 iterations of each of the first 64 primes >= 20,000,000
                           and of each of the 200 uneven exponents (splitmix64 seed 1234,
                           log-uniform primes in [1e6, 6e7]; gpuowl_amd.workloads.uneven200)
+  p<E>_<prp|ll>_50000     the 50,000-iteration serving slices of the first 8 Poisson requests
+                          (seed 7) with exponents in [1e7, 6e7)
 Usage: python tests/golden/make_goldens.py [--jobs N] [--set all|uneven200] [names...]"""
 import argparse, os, subprocess, sys, concurrent.futures as cf
 
@@ -70,7 +72,18 @@ def jobs():
         J[f"p{p}_prp"] = (p, 0, 1000, 500)
     for p in uneven200():
         J[f"p{p}_prp"] = (p, 0, 1000, 500)
+    for E, kind in serving_slices():
+        J[f"p{E}_{kind}_50000"] = (E, int(kind == "ll"), 50_000, 50_000)
     return J
+
+
+def serving_slices(count=8, full_below=10_000_000, gmp_below=60_000_000):
+    """The 50,000-iteration slices among the first `count` serving requests (splitmix64 seed 7)
+    whose GMP run stays within a few core-hours (exponents below gmp_below)."""
+    sys.path.insert(0, ROOT)
+    from gpuowl_amd import workloads
+    return [(E, kind) for _, E, kind in workloads.poisson_requests(count, 1.0)
+            if full_below <= E < gmp_below]
 
 
 def uneven200():
@@ -101,7 +114,7 @@ def run(name, E, ll, iters, step):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
-    ap.add_argument("--set", default="all", choices=["all", "uneven200"])
+    ap.add_argument("--set", default="all", choices=["all", "uneven200", "serving"])
     ap.add_argument("--missing", action="store_true", help="only fixtures not written yet")
     ap.add_argument("names", nargs="*")
     a = ap.parse_args()
@@ -110,6 +123,8 @@ if __name__ == "__main__":
         names = a.names
     elif a.set == "uneven200":
         names = [f"p{p}_prp" for p in uneven200()]
+    elif a.set == "serving":
+        names = [f"p{E}_{kind}_50000" for E, kind in serving_slices()]
     else:
         names = list(all_jobs)
     if a.missing:

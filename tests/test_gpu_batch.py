@@ -98,3 +98,43 @@ def test_uniform_batch_64x20m_vs_goldens():
     for i in have:
         assert rows[i] == [gold[i][500], gold[i][1000]], f"job {i} (p = {exps[i]})"
         assert out[i].nErrors == 0 and out[i].maxRoe < 0.4
+
+
+@pytest.mark.slow
+def test_uneven_200_batch_vs_goldens():
+    # the 200 uneven exponents (splitmix64 seed 1234, log-uniform primes in [1e6, 6e7]) in one batch
+    # at the batch tuner's splits: two Gerbicz blocks of 500 and a verification per job, res64 at
+    # 500 and 1,000 against GMP
+    exps = workloads.uneven200()
+    gold = [load_golden(f"p{p}_prp") for p in exps]
+    missing = [p for p, g in zip(exps, gold) if not g]
+    if missing:
+        pytest.skip(f"{len(missing)} goldens of the uneven exponents not generated")
+    announce("uneven 200-exponent batch", len(exps))
+    owl.tune_batch(exps, "", iters=20)
+    try:
+        with owl.Batch(exps, "") as b:
+            groups = b.n_groups
+            out, rows = b.prp(1000, block_size=500, check_step=1000, res_step=500)
+    finally:
+        owl.tune_batch_clear()
+    assert groups >= 10
+    for i, p in enumerate(exps):
+        assert rows[i] == [gold[i][500], gold[i][1000]], f"job {i} (p = {p})"
+        assert out[i].nErrors == 0 and out[i].nChecks >= 1 and out[i].maxRoe < 0.4, (p, out[i].maxRoe)
+
+
+@pytest.mark.slow
+def test_batch_full_test_checks_past_the_end():
+    # a full PRP test in a batch (iters == E = 859,433, not a multiple of the check step): the
+    # batch runs on to the next block boundary past E and is Gerbicz-checked there before x_E
+    # counts; M859433 is prime, so x_E = 9 (and a second job of a composite exponent beside it
+    # reports its own x_E)
+    E, other = 859_433, 859_447
+    announce("batch full test", E, other)
+    with owl.Batch([E, other], "256:2:256") as b:
+        out, _ = b.prp(E, block_size=1000, check_step=10_000)
+    assert out[0].k == E and out[0].isPrime and out[0].res64 == 9 and out[0].nErrors == 0
+    assert out[1].k == E and not out[1].isPrime and out[1].nErrors == 0
+    # the last check came after E: checks at 10,000 ... 850,000 and at 860,000
+    assert out[0].nChecks == E // 10_000 + 1

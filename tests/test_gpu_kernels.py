@@ -118,3 +118,30 @@ def test_mul_pass(spec, bpw, variant):
         fa = g.read_freq()
         g.run_pass(g.PASS_MUL)
         assert rel(g.read_freq(), m.col_square(fa, mul_spec=m.spectrum(fb))) < 1e-12
+
+
+# Round-off pin: the state is set to H times the forward row transform of digits d + delta*e_k
+# (small digits, so the FFTs are accurate to ~1e-12); the row pass then sees word k at d_k + delta
+# and every other word at an integer, so the round-off it reports must be |delta| -- whatever
+# block, column and word k is in (the redo of block 0, the last word of a block, ...).
+@pytest.mark.parametrize("spec,variant", [("256:2:256", "default"), ("1K:3:256", "default"),
+                                          ("1K:15:256", "default"), ("1K:15:256", "generic")])
+def test_round_off_is_reported_exactly(spec, variant):
+    g, m, rng = setup(spec, 8.0, variant, seed=7)
+    with g:
+        d = m.random_digits(rng).astype(np.float64)
+        N = m.N
+        for k, delta in [(0, 0.3125), (N - 1, -0.4375), (2 * g.R * 7 + 1, 0.28125), (N // 2 + 5, -0.15625)]:
+            for pas in (g.PASS_FUSED, g.PASS_CARRYOUT):
+                y = d.copy()
+                y[k] += delta
+                g.write_freq(m.H * m.lead_in(y))
+                g.read_roe()  # clears the slots
+                g.run_pass(pas, owl.AUX if pas == g.PASS_CARRYOUT else 0)
+                roe = g.read_roe()
+                assert roe["n"] == 1 and abs(roe["max"] - abs(delta)) < 1e-6, (k, delta, pas, roe)
+        # an integer state reports (almost) no round-off
+        g.write_freq(m.H * m.lead_in(d))
+        g.read_roe()
+        g.run_pass(g.PASS_FUSED)
+        assert g.read_roe()["max"] < 1e-6

@@ -137,3 +137,64 @@ def test_fault_in_one_request_is_caught_and_recovered():
         assert r["status"] == 0, r
         assert r["nErrors"] == (1 if rid == faulty else 0), r
         assert r["res64"] == expected(E, k, n), (E, k, n)
+
+
+def load_golden(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"res_{name}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip() and not line.startswith("#"):
+            k, r = line.split()
+            last = (int(k), int(r, 16))
+    return last
+
+
+@pytest.mark.slow
+def test_serving_policy_prefix_matches_alone():
+    # The first 8 requests of the serving workload (Poisson arrivals, splitmix64 seed 7, exponents
+    # log-uniform primes in [2e6, 1.4e8]) under its policy: below 10,000,000 a request runs to
+    # completion (E or E - 2 iterations), otherwise one 50,000-iteration slice; Gerbicz blocks of
+    # 1,000 checked every 50,000 and past the end. Requests are submitted as the server's clock
+    # passes their arrival. Every result equals the same request run alone, and GMP where a
+    # golden covers it (the slices below 6e7).
+    reqs = workloads.poisson_requests(8, 1.0)
+    length = [(E - 2 if kind == "ll" else E) if E < 10_000_000 else 50_000 for _, E, kind in reqs]
+    assert any(n > 50_000 for n in length)  # the prefix holds a full run
+    announce("serving policy prefix", [(E, k, n) for (_, E, k), n in zip(reqs, length)])
+    got, ids = {}, {}
+    with owl.Server(block_size=1000, check_step=50_000) as s:
+        t0, i = s.now(), 0
+        while i < len(reqs) or s.info()["active"] or s.info()["waiting"]:
+            while i < len(reqs) and reqs[i][0] <= s.now() - t0:
+                _, E, kind = reqs[i]
+                ids[s.submit(E, kind, length[i])] = (E, kind, length[i])
+                i += 1
+            if not s.info()["active"] and not s.info()["waiting"]:
+                # nothing to run before the next arrival: jump the clock by submitting it now
+                _, E, kind = reqs[i]
+                ids[s.submit(E, kind, length[i])] = (E, kind, length[i])
+                i += 1
+            for r in s.step():
+                got[r["id"]] = r
+    assert set(got) == set(ids)
+    for rid, (E, kind, n) in ids.items():
+        r = got[rid]
+        assert r["status"] == 0 and r["nErrors"] == 0 and r["maxRoe"] < 0.4, r
+        if kind == "prp":
+            assert r["nChecks"] >= 1
+        announce("alone", E, kind, n)
+        assert r["res64"] == alone_policy(E, kind, n), (E, kind, n)
+        gold = load_golden(f"p{E}_{kind}_50000") if n == 50_000 else None
+        if gold:
+            assert gold == (50_000, r["res64"]), (E, kind)
+
+
+def alone_policy(E, kind, iters):
+    with owl.Gpu(E) as g:
+        if kind == "ll":
+            out, _ = g.ll(iters)
+        else:
+            out, _ = g.prp(iters, block_size=1000, check_step=50_000)
+        return out.res64
