@@ -222,12 +222,15 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   J.ticket = alloc<uint64_t>(1);
   J.roe = alloc<uint32_t>(OWL_ROE_SIZE);
   J.flags = alloc<uint32_t>(8);
+  const size_t nMap = (size_t(nb) * W + 63) / 64, nSum = (nMap + 63) / 64;
+  J.fixMap = alloc<uint64_t>(nMap + nSum);
   J.small = alloc<int32_t>(64);
   J.sum = alloc<uint64_t>(1);
   owlCheck(owl_hip_fill32(queue_, J.stair, 0, 2 * nb * W), "fill");
   owlCheck(owl_hip_fill32(queue_, J.ticket, 0, 2), "fill");
   owlCheck(owl_hip_fill32(queue_, J.roe, 0, OWL_ROE_SIZE), "fill");
   owlCheck(owl_hip_fill32(queue_, J.flags, 0, 8), "fill");
+  owlCheck(owl_hip_fill32(queue_, J.fixMap, 0, 2 * (nMap + nSum)), "fill");
   for (int i = 0; i < 4; ++i) { owlCheck(owl_hip_fill32(queue_, J.words[i], 0, 2 * ND), "fill"); }
   jobs_ = Buffer<JobDesc>(1);
   jobs_.write(queue_, {J});
@@ -325,8 +328,9 @@ u64 Gpu::deviceBytes() const {
 u64 Gpu::footprintBound(u32 N, u32 W, u32 H) {
   const u64 ND = N / 2;
   // 3 spectra (16 B) + 4 word buffers (8 B) per complex point; carries and stairs
-  // (3 x 8 B per column and block, at most ND); weights; round-off, flags, residue
-  return ND * (3 * 16 + 4 * 8) + 3 * 8 * ND + u64(H) * sizeof(LineW) + u64(W) * sizeof(ColW) +
+  // (3 x 8 B per column and block, at most ND); the carry fix map (one bit per column and
+  // block, plus a bit per map word); weights; round-off, flags, residue
+  return ND * (3 * 16 + 4 * 8) + 3 * 8 * ND + ND / 8 + ND / 256 + 16 + u64(H) * sizeof(LineW) + u64(W) * sizeof(ColW) +
          4 * OWL_ROE_SIZE + 4096;
 }
 
@@ -357,7 +361,7 @@ void Gpu::rowCarryOut(Buf out, bool ll, bool mul3, u32 roeSlot, Launch L) {
 void Gpu::carryAll(Buf b, Launch L) {
   int wi = b, ci = 0, co = 1;
   kCarryB(L.q, (nb * W + 255) / 256, L.n, 256, 0, {&L.jobs, &wi, &ci, &co});
-  kCarryFix(L.q, 1, L.n, 64, 0, {&L.jobs, &wi, &co});
+  kCarryFix(L.q, 1, L.n, 256, 0, {&L.jobs, &wi, &co});
 }
 
 // Pairs per column group for a launch of nJobs jobs: the largest power of two up to
@@ -483,20 +487,26 @@ void Gpu::writeIn(Buf b, const Words& words) { writeDigits(b, expandBits(words, 
 
 std::vector<int> Gpu::readDigits(Buf b) {
   Buffer<int32_t> seq(N);
+  std::vector<int> out(N);
+  readDigitsVia(b, seq.get(), out.data());
+  return out;
+}
+
+// k_words_out stages the N digits at `staging` (device), k_sum64 checksums the words buffer; the
+// digits are copied to `out` (host) and accepted when their sum matches (src/Gpu.cpp:677-705).
+void Gpu::readDigitsVia(Buf b, int32_t* staging, int32_t* out) {
   const JobDesc* j = jobs_.get();
   int wi = b;
-  int32_t* p = seq.get();
   for (int attempt = 0; attempt < 3; ++attempt) {
     owlCheck(owl_hip_fill32(queue_, job_.sum, 0, 2), "fill");
     kSum64(queue_, 256, 1, 256, 0, {&j, &wi});
-    kWordsOut(queue_, (ND + 255) / 256, 1, 256, 0, {&j, &wi, &p});
-    std::vector<int> out(N);
+    kWordsOut(queue_, (ND + 255) / 256, 1, 256, 0, {&j, &wi, &staging});
     u64 gpuSum = 0;
     owlCheck(owl_hip_read(queue_, &gpuSum, job_.sum, 8, 0), "read");
-    owlCheck(owl_hip_read(queue_, out.data(), seq.get(), N * 4, 1), "read");
+    owlCheck(owl_hip_read(queue_, out, staging, size_t(N) * 4, 1), "read");
     u64 hostSum = 0;
     for (u32 k = 0; k < N; k += 2) { hostSum += u32(out[k]) | (u64(u32(out[k + 1])) << 32); }
-    if (hostSum == gpuSum) { return out; }
+    if (hostSum == gpuSum) { return; }
   }
   throw GpuError("persistent GPU read errors");
 }

@@ -228,6 +228,7 @@ private:
 
   template<typename T> T* alloc(size_t n);
   template<typename T> T* upload(const std::vector<T>& v);
+  void readDigitsVia(Buf b, int32_t* staging, int32_t* out);
 };
 
 // Mersenne-prime type-1 PRP residue is 9 (src/Gpu.cpp:1036-1040).

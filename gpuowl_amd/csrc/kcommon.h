@@ -278,12 +278,17 @@ __device__ __forceinline__ uint64_t pollCarry(const JobDesc& J, uint64_t* p, uin
 }
 
 // The fused row pass orders its groups by start, not by blockIdx: each group takes the
-// next ticket of the job's counter, and ticket mod (nb + 1) is its block. A group then
-// only waits for groups that have already started, so the carry hand-off progresses even
-// when other kernels (other queues) hold the rest of the device. Every fused launch takes
-// exactly nb + 1 tickets, so the 64-bit counter never needs a reset.
+// next ticket of the job's counter, and the ticket is its block. A group then only waits
+// for groups that have already started, so the carry hand-off progresses even when other
+// kernels (other queues) hold the rest of the device. Every fused launch takes exactly
+// nb + 1 tickets; the group that takes the last one (no other group of the launch touches
+// the counter after it) sets the counter back to 0 for the next launch.
 __device__ __forceinline__ int fusedBlock(const JobDesc& J, int* slot) {
-  if (threadIdx.x == 0) { *slot = int(atomicAdd((unsigned long long*)J.ticket, 1ull) % (unsigned long long)(J.nb + 1)); }
+  if (threadIdx.x == 0) {
+    const int t = int(atomicAdd((unsigned long long*)J.ticket, 1ull) % (unsigned long long)(J.nb + 1));
+    if (t == (int)J.nb) { __hip_atomic_store(J.ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    *slot = t;
+  }
   __syncthreads();
   return *slot;
 }

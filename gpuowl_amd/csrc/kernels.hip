@@ -336,10 +336,42 @@ __global__ __launch_bounds__(T) void k_col(const JobDesc* __restrict__ jobs, int
 }
 
 // ---------------------------------------------------------------- carry propagation
+// Word order of the (line block, column) carries: rank = c * nb + lb, the block of lines
+// [lb*R, lb*R + R) in column c holds words 2*(l + H*c) + rep, so consecutive ranks are
+// consecutive runs of 2R words and rank nb*W - 1 wraps around to rank 0.
+
+// Adds one carry into the 2R words of the block at `rank` and onwards in word order until it
+// dies (or has gone once around); the digits stay balanced.
+__device__ void rippleCarry(const JobDesc& J, int2* words, uint32_t rank, int64_t carry) {
+  const uint32_t W = J.W, R = J.R, nb = J.nb, total = nb * W;
+  for (uint32_t n = 0; carry && n < total; ++n, rank = (rank + 1 == total) ? 0 : rank + 1) {
+    const uint32_t lb = rank % nb, c = rank / nb;
+    int2* blk = words + (size_t)lb * W * R + (size_t)c * R;
+    const ColW cw = J.colW[c];
+    for (uint32_t r = 0; r < R && carry; ++r) {
+      const LineW lw = J.lineW[lb * R + r];
+      int2 d = blk[r];
+      const WordW w0 = wordWeight(J, lw, cw, 0);
+      int64_t x = (int64_t)d.x + carry;
+      int64_t nd = lowBits64(x, w0.bits);
+      carry = (x - nd) >> w0.bits;
+      d.x = (int32_t)nd;
+      if (carry) {
+        const WordW w1 = wordWeight(J, lw, cw, 1);
+        x = (int64_t)d.y + carry;
+        nd = lowBits64(x, w1.bits);
+        carry = (x - nd) >> w1.bits;
+        d.y = (int32_t)nd;
+      }
+      blk[r] = d;
+    }
+  }
+}
+
 // One thread per (line block, column): add the carry-out of the preceding block
 // (in word order) and propagate through this block's 2R words. A carry still
-// left after the block goes to carry[co] and raises flags[0]; the host repeats
-// with the buffers swapped until no carry is left.
+// left after the block goes to carry[co], raises flags[0] and marks its rank in the
+// job's fix map (bit rank of map[], bit rank/64 of the summary after it) for k_carry_fix.
 __global__ __launch_bounds__(256) void k_carryb(const JobDesc* __restrict__ jobs, int wi, int ci, int co) {
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, nb = J.nb;
@@ -371,46 +403,54 @@ __global__ __launch_bounds__(256) void k_carryb(const JobDesc* __restrict__ jobs
     }
   }
   J.carry[co][(size_t)lb * W + c] = carry;
-  if (carry) { atomicOr(J.flags, 1u); }
+  if (carry) {
+    const uint32_t rank = (uint32_t)c * nb + lb;
+    const uint32_t nMap = (nb * W + 63) / 64;
+    atomicOr((unsigned long long*)J.fixMap + rank / 64, 1ull << (rank % 64));
+    atomicOr((unsigned long long*)J.fixMap + nMap + rank / 4096, 1ull << ((rank / 64) % 64));
+    atomicOr(J.flags, 1u);
+  }
 }
 
-// Completes, in word order and on one thread, the rare carries that k_carryb could not
-// absorb inside their block (flags[0] set). Exits at once when there are none.
-__global__ __launch_bounds__(64) void k_carry_fix(const JobDesc* __restrict__ jobs, int wi, int ci) {
+// Completes, in word order, the rare carries that k_carryb could not absorb inside their
+// block (flags[0] set); exits at once when there are none. The group reads the fix map's
+// summary 256 words at a time (clearing it), and thread 0 visits only the marked ranks,
+// in ascending order, rippling each carry onwards; map and flag are left cleared.
+__global__ __launch_bounds__(256) void k_carry_fix(const JobDesc* __restrict__ jobs, int wi, int ci) {
   const JobDesc& J = jobs[blockIdx.y];
-  if (threadIdx.x != 0) { return; }
+  __shared__ unsigned long long sum[256];
   if (__hip_atomic_load(J.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) { return; }
-  J.flags[0] = 0;
-  const uint32_t W = J.W, R = J.R, nb = J.nb;
-  const uint32_t total = nb * W;  // blocks in word order: rank = c * nb + lb
-  for (uint32_t t = 0; t < total; ++t) {
-    uint32_t lb = t % nb, c = t / nb;
-    int64_t carry = J.carry[ci][(size_t)lb * W + c];
-    if (!carry) { continue; }
-    J.carry[ci][(size_t)lb * W + c] = 0;
-    for (uint32_t u = (t + 1) % total, n = 0; carry && n < total; u = (u + 1) % total, ++n) {
-      uint32_t ulb = u % nb, uc = u / nb;
-      int2* blk = J.words[wi] + (size_t)ulb * W * R + (size_t)uc * R;
-      const ColW cw = J.colW[uc];
-      for (uint32_t r = 0; r < R && carry; ++r) {
-        const LineW lw = J.lineW[ulb * R + r];
-        int2 d = blk[r];
-        WordW w0 = wordWeight(J, lw, cw, 0);
-        int64_t x = (int64_t)d.x + carry;
-        int64_t nd = lowBits64(x, w0.bits);
-        carry = (x - nd) >> w0.bits;
-        d.x = (int32_t)nd;
-        if (carry) {
-          WordW w1 = wordWeight(J, lw, cw, 1);
-          x = (int64_t)d.y + carry;
-          nd = lowBits64(x, w1.bits);
-          carry = (x - nd) >> w1.bits;
-          d.y = (int32_t)nd;
+  const uint32_t nb = J.nb, W = J.W, total = nb * W;
+  const uint32_t nMap = (total + 63) / 64, nSum = (nMap + 63) / 64;
+  unsigned long long* map = (unsigned long long*)J.fixMap;
+  for (uint32_t s0 = 0; s0 < nSum; s0 += 256) {
+    const uint32_t i = s0 + threadIdx.x;
+    sum[threadIdx.x] = 0;
+    if (i < nSum) {
+      sum[threadIdx.x] = map[nMap + i];
+      if (sum[threadIdx.x]) { map[nMap + i] = 0; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t k = 0; k < 256 && s0 + k < nSum; ++k) {
+        for (unsigned long long bits = sum[k]; bits; bits &= bits - 1) {
+          const uint32_t mw = (s0 + k) * 64 + __builtin_ctzll(bits);
+          unsigned long long m = map[mw];
+          map[mw] = 0;
+          for (; m; m &= m - 1) {
+            const uint32_t rank = mw * 64 + __builtin_ctzll(m);
+            const uint32_t lb = rank % nb, c = rank / nb;
+            int64_t* slot = J.carry[ci] + (size_t)lb * W + c;
+            const int64_t carry = *slot;
+            *slot = 0;
+            rippleCarry(J, J.words[wi], rank + 1 == total ? 0 : rank + 1, carry);
+          }
         }
-        blk[r] = d;
       }
     }
+    __syncthreads();
   }
+  if (threadIdx.x == 0) { J.flags[0] = 0; }
 }
 
 // ---------------------------------------------------------------- state I/O
@@ -494,7 +534,7 @@ extern "C" const OwlKernelEntry owl_kernel_table[] = {
     {"col256", (const void*)&k_col<256>, 256},
     {"col512", (const void*)&k_col<512>, 512},
     {"carryB", (const void*)&k_carryb, 256},
-    {"carryFix", (const void*)&k_carry_fix, 64},
+    {"carryFix", (const void*)&k_carry_fix, 256},
     {"wordsIn", (const void*)&k_words_in, 256},
     {"wordsOut", (const void*)&k_words_out, 256},
     {"readResidue", (const void*)&k_residue, 64},

@@ -523,15 +523,16 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
 
 // ---------------------------------------------------------------- row pass
 // Weight data of one line read through the scalar cache: the line index is uniform across the
-// group (readfirstlane), so the loads are scalar and the values live in SGPRs.
+// group (readfirstlane), so the loads are scalar and the values live in SGPRs. Vector loads of
+// them would also sit in the in-order vector memory counter behind a prefetch still in flight.
 #define CONSTANT __attribute__((address_space(4)))
 struct LineWs {
-  double w0, w1;
+  double w0, w1, iw0, iw1;
   uint32_t e0, e1;
 };
 __device__ __forceinline__ LineWs lineWs(const LineW* p) {
   const CONSTANT LineW* q = (const CONSTANT LineW*)p;
-  return {q->w[0], q->w[1], q->e[0], q->e[1]};
+  return {q->w[0], q->w[1], q->iw[0], q->iw[1], q->e[0], q->e[1]};
 }
 // Bits of word (line, column, rep) and whether its weight exponent wrapped (see wordWeight).
 __device__ __forceinline__ int wordBits(const JobDesc& J, uint32_t le, uint32_t ce, bool& wrap) {
@@ -571,61 +572,35 @@ __device__ __forceinline__ void pollAll(const JobDesc& J, uint64_t* const* p, ui
   }
 }
 
-// Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 + 64
-// (+ ROWTW_SIZE*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle table.
-// Modes, options and the tagged stairway as k_row.
-template<int W, int R, int F0, int F1, int F2>
-__global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts,
-                                                 uint32_t roeSlot, uint32_t tag) {
-  static_assert(F0 * F1 * F2 == W, "plan");
-  constexpr int T = W / F0;
-  constexpr int LW = R * W;
-  constexpr int REG = rowRegion<W>();
-  extern __shared__ T2 lds[];
-  float* roePart = reinterpret_cast<float*>(lds + R * REG);  // one float per wave, after the line regions
-  T2* twTab = lds + R * REG + 4;                               // after the 64 bytes of round-off slots
-  const JobDesc& J = jobs[blockIdx.y];
-  constexpr bool LDS_TW = W >= 512;
-  using Tw = std::conditional_t<LDS_TW, RowTw, GlobalTw>;
-  const Tw tw{LDS_TW ? twTab : J.twRow};
-  const int nb = J.nb;
-  __shared__ int ticketSlot;
-  // the block index is uniform: line weights then come through the scalar cache
-  const int g = __builtin_amdgcn_readfirstlane((mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x);
-  const int lb = (g == nb) ? 0 : g;
-  const int tid = threadIdx.x;
-  const bool ll = opts & 1, mul3 = opts & 2;
+// Pieces of the row pass over one block of R lines, T = W/F0 threads. Thread tid owns columns
+// c = tid + k*T (k < F0) of the R lines; v[r][k] holds the inverse transform of (line r,
+// column c), then its balanced digits (as doubles), then the weighted digits.
+template<int W, int R, int F0, int F1, int F2> struct RowPass {
+  static constexpr int T = W / F0;
+  static constexpr int LW = R * W;
+  static constexpr int NL = LW / T;  // state elements per thread
+  static constexpr int REG = rowRegion<W>();
+  static constexpr bool LDS_TW = W >= 512;
   // the block streams through with non-temporal accesses from W = 512 up
-  constexpr bool NT = W >= 512;
-  T2* blk = J.freq[fi] + (size_t)lb * LW;
-  if constexpr (LDS_TW) { fillRowTw<W, T>(J.twRow, twTab, tid); }  // read after the first barrier below
+  static constexpr bool NT = W >= 512;
+  using Tw = std::conditional_t<LDS_TW, RowTw, GlobalTw>;
 
-  // Thread tid owns columns c = tid + k*T (k < F0) of the R lines of the block. v holds the
-  // inverse transform, then the balanced digits (as doubles), then the weighted digits.
-  T2 v[R][F0];
-  ColW cw[F0];
+  // element e = s*R + r of the block (slot s, line r) for e = tid + i*T
+  __device__ __forceinline__ static void load(const T2* blk, T2* P, int tid) {
 #pragma unroll
-  for (int k = 0; k < F0; ++k) { cw[k] = gld(J.colW + tid + k * T); }
-  const LineW* lwBase = J.lineW + lb * R;
-
-  if (mode == ROW_LEADIN) {
-    const int2* src = J.words[wi] + (size_t)lb * LW;
+    for (int i = 0; i < NL; ++i) { P[i] = ldx<NT>(blk + tid + i * T); }
+  }
+  // ... into the line regions in frequency order
+  __device__ __forceinline__ static void scatter(T2* lds, const T2* P, int tid) {
 #pragma unroll
-    for (int k = 0; k < F0; ++k) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int2 d = gld(src + (tid + k * T) * R + r);
-        v[r][k] = make_double2(d.x, d.y);
-      }
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < LW / T; ++i) {
+    for (int i = 0; i < NL; ++i) {
       const int e = tid + i * T;
       const int s = e / R, r = e % R;
-      lds[r * REG + padi(slotFreq(s, W))] = ldx<NT>(blk + e);
+      lds[r * REG + padi(slotFreq(s, W))] = P[i];
     }
-    __syncthreads();
+  }
+  // inverse W-point FFT of the R lines (LDS stages, then the radix-F0 stage into registers)
+  __device__ __forceinline__ static void inverse(T2* lds, const Tw& tw, int tid, T2 (&v)[R][F0]) {
     XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, tw, tid);
     __syncthreads();
     // last inverse stage (radix F0, Ns = T): butterfly tid of each line -> columns tid + k*T
@@ -637,31 +612,32 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       for (int k = 1; k < F0; ++k) { v[r][k] = cmulc(v[r][k], tw(tid * k)); }
       Dft<F0, true>::run(v[r]);
     }
-
-    // Unweight, round, carry along the block's 2R words per column. The carry chain runs in
-    // FP64: every value is an integer of magnitude below 2^51, so sums, scaling by powers of two
-    // and floor are exact. Word u of b bits splits into the carry floor(u/2^b + 1/2) and the
-    // balanced digit u - carry*2^b in [-2^(b-1), 2^(b-1)), as the integer chain would.
-    double roe = 0;
-    double cs[F0];
+  }
+  // Unweight, round, carry along the block's 2R words per column. The carry chain runs in
+  // FP64: every value is an integer of magnitude below 2^51, so sums, scaling by powers of two
+  // and floor are exact. Word u of b bits splits into the carry floor(u/2^b + 1/2) and the
+  // balanced digit u - carry*2^b in [-2^(b-1), 2^(b-1)), as the integer chain would.
+  // cs[k] = the block's carry-out of column k; roe = max round-off.
+  __device__ __forceinline__ static void carry(const JobDesc& J, const LineW* lwBase, const ColW* cw, T2 (&v)[R][F0],
+                                               double* cs, double& roe, bool llHere, bool mul3) {
 #pragma unroll
     for (int k = 0; k < F0; ++k) { cs[k] = 0; }
     const double m3 = mul3 ? 3.0 : 1.0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const LineW lw = gld(lwBase + r);
+      const LineWs lw = lineWs(lwBase + r);
 #pragma unroll
       for (int k = 0; k < F0; ++k) {
 #pragma unroll
         for (int rep = 0; rep < 2; ++rep) {
           bool wrap;
-          const int b = wordBits(J, lw.e[rep], cw[k].e, wrap);
-          const double iw = __builtin_amdgcn_ldexp(lw.iw[rep] * cw[k].iw, wrap ? 1 : 0);
+          const int b = wordBits(J, rep ? lw.e1 : lw.e0, cw[k].e, wrap);
+          const double iw = __builtin_amdgcn_ldexp((rep ? lw.iw1 : lw.iw0) * cw[k].iw, wrap ? 1 : 0);
           const double x = rep ? v[r][k].y : v[r][k].x;
           const double t = fma(x, iw, OWL_RNDVAL) - OWL_RNDVAL;
           roe = fmax(roe, fabs(fma(x, iw, -t)));
           double u = fma(t, m3, cs[k]);
-          if (ll && r == 0 && k == 0 && rep == 0 && lb == 0 && tid == 0) { u -= 2; }
+          if (llHere && r == 0 && k == 0 && rep == 0) { u -= 2; }
           const double c = floor(__builtin_amdgcn_ldexp(u, -b) + 0.5);
           const double dgt = u - __builtin_amdgcn_ldexp(c, b);
           cs[k] = c;
@@ -673,45 +649,10 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
         }
       }
     }
-    if (mode == ROW_CARRYOUT) {
-      roeReport(J, roeSlot, (float)roe);
-      int2* dst = J.words[wi] + (size_t)lb * LW;
-#pragma unroll
-      for (int k = 0; k < F0; ++k) {
-        const int c = tid + k * T;
-        gst(J.carry[0] + (size_t)lb * W + c, (int64_t)cs[k]);
-#pragma unroll
-        for (int r = 0; r < R; ++r) { gst(dst + c * R + r, make_int2((int)v[r][k].x, (int)v[r][k].y)); }
-      }
-      return;
-    }
-
-    if (g < nb) {
-#pragma unroll
-      for (int k = 0; k < F0; ++k) {
-        stairStore(J.stair + (size_t)g * W + tid + k * T, packCarry((int64_t)cs[k], tag));
-      }
-    }
-    if (g == 0) {
-      roeReport(J, roeSlot, (float)roe);
-      return;
-    }
-    roeWave(roeSlot, (float)roe, roePart);
-    uint64_t* src[F0];
-    uint64_t got[F0];
-#pragma unroll
-    for (int k = 0; k < F0; ++k) {
-      const int c = tid + k * T;
-      src[k] = J.stair + (size_t)(g - 1) * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
-    }
-    pollAll<F0>(J, src, tag, got);
-    if (g == nb) {  // group 0 has read block 0
-#pragma unroll
-      for (int k = 0; k < F0; ++k) { src[k] = J.stair + tid + k * T; }
-      uint64_t seen[F0];
-      pollAll<F0>(J, src, tag, seen);
-    }
-    // ripple the predecessor's carry through the column's 2R words until it dies
+  }
+  // Ripples the predecessor's carries through each column's 2R words until they die.
+  __device__ __forceinline__ static void ripple(const JobDesc& J, const LineW* lwBase, const ColW* cw, T2 (&v)[R][F0],
+                                                const uint64_t* got) {
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
       double carry = (double)unpackCarry(got[k]);
@@ -737,38 +678,147 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       }
     }
   }
-
-  // weight, then forward stage 0 (radix F0, Ns = 1) straight from registers
+  // Weight, then the forward W-point FFT's stage 0 (radix F0, Ns = 1), in registers.
+  __device__ __forceinline__ static void weightStage0(const JobDesc& J, const LineW* lwBase, const ColW* cw,
+                                                      T2 (&v)[R][F0]) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const LineWs lw = lineWs(lwBase + r);
+    for (int r = 0; r < R; ++r) {
+      const LineWs lw = lineWs(lwBase + r);
+#pragma unroll
+      for (int k = 0; k < F0; ++k) {
+        bool wrap0, wrap1;
+        wordBits(J, lw.e0, cw[k].e, wrap0);
+        wordBits(J, lw.e1, cw[k].e, wrap1);
+        const double w0 = __builtin_amdgcn_ldexp(lw.w0 * cw[k].w, wrap0 ? -1 : 0);
+        const double w1 = __builtin_amdgcn_ldexp(lw.w1 * cw[k].w, wrap1 ? -1 : 0);
+        v[r][k] = make_double2(v[r][k].x * w0, v[r][k].y * w1);
+      }
+      Dft<F0, false>::run(v[r]);
+    }
+  }
+  // The other forward stages in LDS. The caller has made sure that no thread still reads the
+  // line regions.
+  __device__ __forceinline__ static void forwardLds(const T2 (&v)[R][F0], T2* lds, const Tw& tw, int tid) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int k = 0; k < F0; ++k) { lds[r * REG + padi(tid * F0 + k)] = v[r][k]; }
+    }
+    __syncthreads();
+    XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, tw, tid);
+    __syncthreads();
+  }
+  // the block from the line regions, slot order
+  __device__ __forceinline__ static void store(T2* blk, const T2* lds, int tid) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = tid + i * T;
+      const int s = e / R, r = e % R;
+      stx<NT>(blk + e, lds[r * REG + padi(slotFreq(s, W))]);
+    }
+  }
+  // the stairway carries this block (g) waits for: block g-1's, shifted by one column for the
+  // redo of block 0 (g == nb, the wrap-around)
+  __device__ __forceinline__ static void pollPredecessor(const JobDesc& J, int g, int tid, uint32_t tag, uint64_t* got) {
+    const int nb = J.nb;
+    uint64_t* src[F0];
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
-      bool wrap0, wrap1;
-      wordBits(J, lw.e0, cw[k].e, wrap0);
-      wordBits(J, lw.e1, cw[k].e, wrap1);
-      const double w0 = __builtin_amdgcn_ldexp(lw.w0 * cw[k].w, wrap0 ? -1 : 0);
-      const double w1 = __builtin_amdgcn_ldexp(lw.w1 * cw[k].w, wrap1 ? -1 : 0);
-      v[r][k] = make_double2(v[r][k].x * w0, v[r][k].y * w1);
+      const int c = tid + k * T;
+      src[k] = J.stair + (size_t)(g - 1) * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
     }
-    Dft<F0, false>::run(v[r]);
+    pollAll<F0>(J, src, tag, got);
+    if (g == nb) {  // block 0's first visit has read block 0 before the redo rewrites it
+#pragma unroll
+      for (int k = 0; k < F0; ++k) { src[k] = J.stair + tid + k * T; }
+      uint64_t seen[F0];
+      pollAll<F0>(J, src, tag, seen);
+    }
   }
+  __device__ __forceinline__ static void publish(const JobDesc& J, int g, int tid, uint32_t tag, const double* cs) {
+#pragma unroll
+    for (int k = 0; k < F0; ++k) { stairStore(J.stair + (size_t)g * W + tid + k * T, packCarry((int64_t)cs[k], tag)); }
+  }
+};
+
+// Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 + 64
+// (+ ROWTW_SIZE*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle table.
+// Modes, options and the tagged stairway as k_row.
+template<int W, int R, int F0, int F1, int F2>
+__global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts,
+                                                 uint32_t roeSlot, uint32_t tag) {
+  static_assert(F0 * F1 * F2 == W, "plan");
+  using P = RowPass<W, R, F0, F1, F2>;
+  constexpr int T = P::T, LW = P::LW, REG = P::REG;
+  extern __shared__ T2 lds[];
+  float* roePart = reinterpret_cast<float*>(lds + R * REG);  // one float per wave, after the line regions
+  T2* twTab = lds + R * REG + 4;                               // after the 64 bytes of round-off slots
+  const JobDesc& J = jobs[blockIdx.y];
+  const typename P::Tw tw{P::LDS_TW ? twTab : J.twRow};
+  const int nb = J.nb;
+  __shared__ int ticketSlot;
+  // the block index is uniform: line weights then come through the scalar cache
+  const int g = __builtin_amdgcn_readfirstlane((mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x);
+  const int lb = (g == nb) ? 0 : g;
+  const int tid = threadIdx.x;
+  const bool ll = opts & 1, mul3 = opts & 2;
+  T2* blk = J.freq[fi] + (size_t)lb * LW;
+  if constexpr (P::LDS_TW) { fillRowTw<W, T>(J.twRow, twTab, tid); }  // read after the first barrier below
+
+  T2 v[R][F0];
+  ColW cw[F0];
+#pragma unroll
+  for (int k = 0; k < F0; ++k) { cw[k] = gld(J.colW + tid + k * T); }
+  const LineW* lwBase = J.lineW + lb * R;
+
+  if (mode == ROW_LEADIN) {
+    const int2* src = J.words[wi] + (size_t)lb * LW;
+#pragma unroll
+    for (int k = 0; k < F0; ++k) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int2 d = gld(src + (tid + k * T) * R + r);
+        v[r][k] = make_double2(d.x, d.y);
+      }
+    }
+  } else {
+    {
+      T2 x[P::NL];
+      P::load(blk, x, tid);
+      P::scatter(lds, x, tid);
+    }
+    __syncthreads();
+    P::inverse(lds, tw, tid, v);
+    double roe = 0;
+    double cs[F0];
+    P::carry(J, lwBase, cw, v, cs, roe, ll && lb == 0 && tid == 0, mul3);
+    if (mode == ROW_CARRYOUT) {
+      roeReport(J, roeSlot, (float)roe);
+      int2* dst = J.words[wi] + (size_t)lb * LW;
+#pragma unroll
+      for (int k = 0; k < F0; ++k) {
+        const int c = tid + k * T;
+        gst(J.carry[0] + (size_t)lb * W + c, (int64_t)cs[k]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) { gst(dst + c * R + r, make_int2((int)v[r][k].x, (int)v[r][k].y)); }
+      }
+      return;
+    }
+    if (g < nb) { P::publish(J, g, tid, tag, cs); }
+    if (g == 0) {
+      roeReport(J, roeSlot, (float)roe);
+      return;
+    }
+    roeWave(roeSlot, (float)roe, roePart);
+    uint64_t got[F0];
+    P::pollPredecessor(J, g, tid, tag, got);
+    P::ripple(J, lwBase, cw, v, got);
+  }
+  P::weightStage0(J, lwBase, cw, v);
   __syncthreads();  // every thread has finished reading the inverse's LDS
   if (mode == ROW_FUSED) { roeGroup(J, roeSlot, roePart, T / 64); }
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-#pragma unroll
-    for (int k = 0; k < F0; ++k) { lds[r * REG + padi(tid * F0 + k)] = v[r][k]; }
-  }
-  __syncthreads();
-  XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, tw, tid);
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < LW / T; ++i) {
-    const int e = tid + i * T;
-    const int s = e / R, r = e % R;
-    stx<NT>(blk + e, lds[r * REG + padi(slotFreq(s, W))]);
-  }
+  P::forwardLds(v, lds, tw, tid);
+  P::store(blk, lds, tid);
 }
 
 // ---------------------------------------------------------------- name tables

@@ -225,6 +225,19 @@ def test_long_carry_mode_matches():
         assert res == oracle.gmp_res64(E, 60, 20)
 
 
+@pytest.mark.parametrize("spec,bpw", [("64:2:64", 3.0), ("256:3:256", 3.1)])
+def test_carries_past_whole_blocks_match(spec, bpw):
+    # one line per block and a full carry every iteration at ~3 bits per word: carries often
+    # cross whole 2-word blocks, so the carry fix (the marked ranks, in word order) runs on most
+    # iterations
+    E, N = exponent_for(spec, bpw)
+    announce("carries past blocks", spec, E)
+    with owl.Gpu(E, spec, long_carry=True, line_block=1) as g:
+        out, res = g.prp(100, block_size=20, check_step=20, res_step=20)
+        assert res == oracle.gmp_res64(E, 100, 20)
+        assert out.nErrors == 0
+
+
 @pytest.mark.slow
 def test_mersenne_prime_1257787():
     # M1257787 is prime: the type-1 PRP residue after E squarings is 9
