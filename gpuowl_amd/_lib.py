@@ -99,6 +99,7 @@ SIGNATURES = {
     "owl_gpu_read_words": [vp, i32, P(u32), u32, P(i32)],
     "owl_gpu_write_digits": [vp, i32, P(i32), u32],
     "owl_gpu_read_digits": [vp, i32, P(i32), u32],
+    "owl_gpu_read_digits_noalloc": [vp, i32, P(i32), u32],
     "owl_gpu_res64": [vp, i32, P(u64)],
     "owl_gpu_square_loop": [vp, i32, i32, u32, u32, i32, i32],
     "owl_gpu_mod_mul": [vp, i32, i32, i32],

@@ -492,6 +492,16 @@ std::vector<int> Gpu::readDigits(Buf b) {
   return out;
 }
 
+void Gpu::readDigitsNoAlloc(Buf b, int32_t* out) {
+  int32_t* staging = nullptr;
+  if (N > OWL_NOALLOC_MAX_WORDS) {
+    throw std::invalid_argument("read_digits_noalloc: N = " + std::to_string(N) + " words exceeds the static staging "
+                                "area of " + std::to_string(OWL_NOALLOC_MAX_WORDS) + " words");
+  }
+  if (owlNoAllocScratch(device, &staging)) { throw GpuError("read_digits_noalloc: no address for the staging area"); }
+  readDigitsVia(b, staging, out);
+}
+
 // k_words_out stages the N digits at `staging` (device), k_sum64 checksums the words buffer; the
 // digits are copied to `out` (host) and accepted when their sum matches (src/Gpu.cpp:677-705).
 void Gpu::readDigitsVia(Buf b, int32_t* staging, int32_t* out) {

@@ -19,6 +19,10 @@
 #include <string>
 #include <vector>
 
+// Device address of the static staging array of the no-allocation digit readout on `device`
+// (resolved once per device, kernels.hip); 0 on success.
+extern "C" int owlNoAllocScratch(int device, int32_t** ptr);
+
 // A named kernel of the offline-compiled library (src/Kernel.h:17-79).
 class Kernel {
   owl_kernel k_ = nullptr;
@@ -94,6 +98,9 @@ public:
   void writeIn(Buf b, const Words& words);
   void writeDigits(Buf b, const std::vector<int>& digits);  // sequential balanced digits
   std::vector<int> readDigits(Buf b);                       // sequential digits, checksum-verified
+  // readDigits into out (N int32, host) through the library's static staging array instead of
+  // a staging buffer allocated per call; rejects N > OWL_NOALLOC_MAX_WORDS
+  void readDigitsNoAlloc(Buf b, int32_t* out);
   Words readData(Buf b = DATA);                             // canonical residue words
   u64 dataResidue(Buf b = DATA);                            // res64 via readResidue (src/Gpu.cpp:955-975)
 

@@ -180,6 +180,13 @@ int owl_gpu_read_digits(owl_gpu g, int b, int32_t* digits, uint32_t n) {
   });
 }
 
+int owl_gpu_read_digits_noalloc(owl_gpu g, int b, int32_t* digits, uint32_t n) {
+  return guarded([&] {
+    if (n != g->gpu->N) { throw std::invalid_argument("digit count mismatch"); }
+    g->gpu->readDigitsNoAlloc(buf(b), digits);
+  });
+}
+
 int owl_gpu_res64(owl_gpu g, int b, uint64_t* res) {
   return guarded([&] { *res = g->gpu->dataResidue(buf(b)); });
 }

@@ -22,6 +22,10 @@
 
 #define OWL_MAX_STAGES 8
 #define OWL_ROE_SIZE 100000
+// Largest N (words) whose digits the no-allocation readout stages in its static array
+// (2^23 int32 = 32 MiB per device; covers FFTs up to 8M words).
+#define OWL_NOALLOC_MAX_WORDS (1u << 23)
+#define OWL_MAX_DEVICES 64
 
 // Per-line IBDWT weight data: words 2*(l + H*c) + rep for rep in {0,1}.
 // e[rep] = step*(2l+rep) mod N; w = 2^(e/N); iw = 2^(-e/N).

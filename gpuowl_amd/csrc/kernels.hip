@@ -544,3 +544,34 @@ extern "C" const OwlKernelEntry owl_kernel_table[] = {
     {"copyWords", (const void*)&k_copy_words, 256},
     {nullptr, nullptr, 0},
 };
+
+// ---------------------------------------------------------------- staging without allocation
+// The digit readout (k_words_out) stages N int32 in device memory. owl_gpu_read_digits
+// allocates that staging buffer per call; owl_gpu_read_digits_noalloc stages through this
+// file-scope array instead, for callers that must not allocate. Each device holds its own
+// copy of the array until the process ends. In host code its name is not a device address:
+// owlNoAllocScratch() resolves the address once per device (with that device current) and
+// kernels only ever receive that pointer. Calls on one device share the array, so
+// overlapping calls on one device race; every call overwrites the N words it reads back.
+static __device__ __align__(256) int32_t scratch[OWL_NOALLOC_MAX_WORDS];
+
+#include <mutex>
+
+extern "C" int owlNoAllocScratch(int device, int32_t** ptr) {
+  static std::mutex mu;
+  static int32_t* addr[OWL_MAX_DEVICES] = {};
+  if (device < 0 || device >= OWL_MAX_DEVICES) { return 1; }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) { return 2; }
+    if (cur != device && hipSetDevice(device) != hipSuccess) { return 2; }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) { (void)hipSetDevice(cur); }
+    if (e != hipSuccess || !p) { return 3; }
+    addr[device] = static_cast<int32_t*>(p);
+  }
+  *ptr = addr[device];
+  return 0;
+}

@@ -104,6 +104,12 @@ class Gpu:
         check(self.lib.owl_gpu_read_digits(self.h, buf, arr, self.N))
         return list(arr)
 
+    def read_digits_noalloc(self, buf=DATA):
+        """read_digits staged through the library's static device array (no device allocation)."""
+        arr = (C.c_int32 * self.N)()
+        check(self.lib.owl_gpu_read_digits_noalloc(self.h, buf, arr, self.N))
+        return list(arr)
+
     def res64(self, buf=DATA):
         r = C.c_uint64()
         check(self.lib.owl_gpu_res64(self.h, buf, C.byref(r)))

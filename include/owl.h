@@ -77,6 +77,12 @@ int owl_gpu_read_words(owl_gpu gpu, int buf, uint32_t* words, uint32_t nWords, i
 /* Raw balanced digits in sequential word order, N int32 (src/Gpu.cpp:677-705, 801-811). */
 int owl_gpu_write_digits(owl_gpu gpu, int buf, const int32_t* digits, uint32_t n);
 int owl_gpu_read_digits(owl_gpu gpu, int buf, int32_t* digits, uint32_t n);
+/* No counterpart in the reference: owl_gpu_read_digits without any device allocation, for callers
+ * that cannot allocate (code under graph capture, serving loops, harnesses that forbid it). The
+ * digits are staged in a static device array of OWL_NOALLOC_MAX_WORDS = 2^23 int32 (32 MiB, held
+ * per device until the process ends); N above that is rejected with an error. Calls on one device
+ * share that array: overlapping calls on one device race. */
+int owl_gpu_read_digits_noalloc(owl_gpu gpu, int buf, int32_t* digits, uint32_t n);
 
 /* Gpu::dataResidue / bufResidue (src/Gpu.cpp:955-975). */
 int owl_gpu_res64(owl_gpu gpu, int buf, uint64_t* res);
