@@ -87,6 +87,7 @@ SIGNATURES = {
     "owl_hip_graph_launch": [vp, vp],
     "owl_hip_graph_destroy": [vp],
     "owl_fft_best_fit": [u32, cp, cp, C.c_size_t, P(u32), P(u32)],
+    "owl_fft_best_fit_ref": [u32, cp, cp, C.c_size_t, P(u32), P(u32)],
     "owl_fft_tune_load": [cp],
     "owl_fft_tune_save": [cp],
     "owl_fft_tune_clear": [],

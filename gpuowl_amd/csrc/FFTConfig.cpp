@@ -21,6 +21,15 @@ const BpwRow BPW_TABLE[] = {
 #include "bpw_table.inc"
 };
 
+struct CapRow {
+  u32 w, m, h;
+  double bpw;
+};
+
+const CapRow ENGINE_CAPS[] = {
+#include "engine_bpw.inc"
+};
+
 // First matching row, as a std::map built from the table keeps the first of duplicate keys.
 const BpwRow* findBpw(u32 w, u32 m, u32 h) {
   for (const BpwRow& r : BPW_TABLE) {
@@ -184,7 +193,16 @@ std::string FFTConfig::spec() const {
   return carry == CARRY_AUTO ? s : s + (carry == CARRY_32 ? ":0" : ":1");
 }
 
-double FFTConfig::maxBpw() const {
+double FFTShape::engineCap() const {
+  for (const CapRow& r : ENGINE_CAPS) {
+    if (r.w == width && r.m == middle && r.h == height) { return r.bpw; }
+  }
+  return HUGE_VAL;
+}
+
+double FFTConfig::maxBpw() const { return std::min(refMaxBpw(), shape.engineCap()); }
+
+double FFTConfig::refMaxBpw() const {
   u32 vW = variant / 100, vM = variant % 100 / 10, vH = variant % 10;
   double b;
   if (vW == vH || (shape.width > 1024 && vW == 1 && vH == 0)) {
@@ -195,16 +213,18 @@ double FFTConfig::maxBpw() const {
   return carry == CARRY_32 ? std::min(shape.carry32BPW(), b) : b;
 }
 
-FFTConfig FFTConfig::bestFit(u32 E, const std::string& spec, double overdrive) {
+FFTConfig FFTConfig::bestFit(u32 E, const std::string& spec, double overdrive, bool refLimits) {
   if (!spec.empty()) { return FFTConfig{spec}; }
   // the cheapest tuned FFT that handles E (the list is ordered by cost)
-  for (const FftTuneEntry& e : FftTune::get()) {
-    if (e.fft.maxExp() * overdrive >= E) { return e.fft; }
+  if (!refLimits) {
+    for (const FftTuneEntry& e : FftTune::get()) {
+      if (e.fft.maxExp() * overdrive >= E) { return e.fft; }
+    }
   }
   for (const FFTShape& shape : FFTShape::allShapes()) {
     for (u32 v : {101u, 202u}) {
       FFTConfig fft{shape, v, CARRY_AUTO};
-      if (fft.maxExp() * overdrive >= E) { return fft; }
+      if ((refLimits ? fft.refMaxExp() : fft.maxExp()) * overdrive >= E) { return fft; }
     }
   }
   throw std::invalid_argument("no FFT for exponent " + std::to_string(E));

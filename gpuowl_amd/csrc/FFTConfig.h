@@ -1,7 +1,8 @@
 // FFT shape and per-exponent FFT selection.
 // Mirrors the reference interface src/FFTConfig.h:24-87 (FFTShape, FFTConfig,
 // bestFit, maxBpw, carry32BPW, needsLargeCarry, isFavoredShape) so that the
-// FFT length chosen for an exponent is the reference's.
+// FFT length chosen for an exponent is the reference's, except at the top of the
+// shapes where this engine's measured round-off calls for a lower limit.
 #pragma once
 
 #include <array>
@@ -33,6 +34,9 @@ public:
   explicit FFTShape(const std::string& spec);
 
   u32 size() const { return width * middle * height * 2; }
+  // this engine's max bits per word for the shape where its kernels' round-off (Gumbel Z = 28)
+  // needs less than the reference's table allows; +infinity elsewhere (engine_bpw.inc)
+  double engineCap() const;
   u32 nW() const { return (width == 1024 || width == 256) ? 4 : 8; }
   u32 nH() const { return (height == 1024 || height == 256) ? 4 : 8; }
   double maxBpw() const;
@@ -51,8 +55,9 @@ constexpr u32 LAST_VARIANT = 212;
 struct FFTConfig {
   // spec empty: the first FFT of the loaded tune file (FftTune) whose maxExp covers E,
   // else the first shape (variants 101 then 202) that does (src/FFTConfig.cpp:235-263).
-  // Otherwise the spec's FFT.
-  static FFTConfig bestFit(u32 E, const std::string& spec, double overdrive = 1.0);
+  // Otherwise the spec's FFT. refLimits: the reference's limits (refMaxExp) instead of this
+  // engine's (maxExp) and no tune file, i.e. the reference's own choice.
+  static FFTConfig bestFit(u32 E, const std::string& spec, double overdrive = 1.0, bool refLimits = false);
 
   FFTShape shape{};
   u32 variant = LAST_VARIANT;
@@ -64,8 +69,12 @@ struct FFTConfig {
 
   std::string spec() const;
   u32 size() const { return shape.size(); }
+  // this engine's limits: the reference's, capped by FFTShape::engineCap()
   u32 maxExp() const { return u32(maxBpw() * shape.size()); }
   double maxBpw() const;
+  // the reference's limits (src/FFTConfig.cpp:218-233)
+  u32 refMaxExp() const { return u32(refMaxBpw() * shape.size()); }
+  double refMaxBpw() const;
 };
 
 // The reference's tune.txt (TuneEntry, src/TuneEntry.cpp:7-84): FFTs ordered by cost with

@@ -142,7 +142,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &rowT), "kernel");
     // R regions of rowRegion<W>() (kfixed.h) + per-wave round-off + ROWTW_SIZE-entry twiddle table (W >= 512)
-    rowLds = R * (W + W / 16 + 4) * 16 + 64 + (W >= 512 ? 128 * 16 : 0);
+    rowLds = R * (W + W / 16 + 4) * 16 + 64 + (W >= 512 ? 132 * 16 : 0);
   } else {
     rowKernel = "row" + std::to_string(rowT);
   }
@@ -151,7 +151,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
     // colRegion<H>() regions + a colTwSize<H>()-entry twiddle table (kfixed.h)
-    const u32 twSize = 64 + H / 64 + 2 * (64 + H / 64 + 16);
+    const u32 twSize = 68 + H / 64 + 2 * (64 + H / 64 + 16);
     colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + twSize) * 16;
     if (useX && !opt.colOnePair && haveKernel(colX + "m")) {
       int cus = 0, lds = 0;

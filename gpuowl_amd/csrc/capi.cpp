@@ -66,6 +66,15 @@ int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen,
   });
 }
 
+int owl_fft_best_fit_ref(uint32_t E, const char* spec, char* outSpec, size_t outLen, uint32_t* N, uint32_t* maxExp) {
+  return guarded([&] {
+    FFTConfig fft = FFTConfig::bestFit(E, spec ? spec : "", 1.0, true);
+    if (outSpec && outLen) { snprintf(outSpec, outLen, "%s", fft.spec().c_str()); }
+    if (N) { *N = fft.size(); }
+    if (maxExp) { *maxExp = fft.refMaxExp(); }
+  });
+}
+
 int owl_fft_tune_load(const char* path) {
   return guarded([&] {
     if (!path) { throw std::invalid_argument("tune file path"); }

@@ -6,7 +6,7 @@
 // loop unrolled) and fewer LDS round trips:
 //   k_colx: the first forward stage is fused with the global load and the mid
 //           twiddle, the last inverse stage with the conjugate twiddle and the
-//           store (forward plan [C0,C1,C2], inverse plan [C2,C1,C0]).
+//           store (forward plan [C0,C1,C2], inverse plan [C1,C2,C0]).
 //   k_rowx: the last inverse stage leaves each thread holding whole columns
 //           (T = W/F0 threads, columns tid + k*T for all R lines), so rounding,
 //           the carry chain, the stairway hand-off and the weighting all run in
@@ -127,37 +127,52 @@ struct GlobalTw {
   __device__ __forceinline__ T2 operator()(int m) const { return gld(t + m); }
 };
 
+// The low-part tables TA[i] = w^i (i < 64) below are read at i = (p*k*stride) & 63 for the lanes
+// p of a wave and a fixed k; for even k many lanes of one LDS access phase would then hit the same
+// banks (k = 8: eight distinct entries i = 0, 8, 16, ... in one bank row of 64 4-byte banks). TA
+// is therefore stored with one pad slot after every 16 entries (entry i at i + i/16, 68 slots),
+// which spreads those entries over the banks (about half the bank cycles of the unpadded table
+// for the row pass's stages, a fifth less for the column pass's).
+__device__ __forceinline__ constexpr int taPos(int i) { return i + (i >> 4); }
+constexpr int TA_SLOTS = 68;
+// table position p of TA (p < TA_SLOTS) -> entry index, -1 for a pad slot
+__device__ __forceinline__ constexpr int taEntry(int p) { return (p % 17 == 16) ? -1 : 16 * (p / 17) + p % 17; }
+
 // Row-pass twiddles staged in LDS (ROWTW_SIZE entries after the line regions and the
 // round-off slots): w_W^m = TA[m & 63] * TB[m >> 6], TA[i] = w_W^i, TB[j] = w_W^(64 j),
 // each twiddle a product of two correctly rounded table entries. Used for W >= 512; at
 // W = 256 the fill (68 entries for a 16 KB block) costs more than the global reads save.
-constexpr int ROWTW_SIZE = 128;
+constexpr int ROWTW_SIZE = TA_SLOTS + 64;
 struct RowTw {
   const T2* tab;
-  __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[m & 63], tab[64 + (m >> 6)]); }
+  __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
 };
 template<int W, int T>
 __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
   static_assert(W % 64 == 0 && W / 64 <= 64, "twiddle table");
-  constexpr int N = 64 + W / 64;
+  constexpr int N = TA_SLOTS + W / 64;
 #pragma unroll
-  for (int i = tid; i < N; i += T) { tab[i] = gld(twRow + (i < 64 ? i : 64 * (i - 64))); }
+  for (int p = tid; p < N; p += T) {
+    const int e = p < TA_SLOTS ? taEntry(p) : 64 * (p - TA_SLOTS);
+    tab[p] = e < 0 ? make_double2(0, 0) : gld(twRow + e);
+  }
 }
 
 // Column-pass twiddles staged in LDS after the data regions (colTwSize<H>() entries):
 // w_H^m = TA[m & 63] * TB[m >> 6] for m < H, and for column h of the pair the mid
 // twiddles w_ND^(a*l) = MA[h][l & 63] * MB[h][l >> 6] and S[h][k] = w_ND^(a*k*H/16).
-// Each twiddle is a product of two correctly rounded table entries. Layout: TA[64],
-// TB[H/64], then per column MA[64], MB[H/64], S[16]; sized per H so that small-H groups
-// keep their LDS footprint (and so the number of groups per CU) as low as the data allows.
+// Each twiddle is a product of two correctly rounded table entries. Layout: TA (72 slots,
+// padded as above), TB[H/64], then per column MA[64], MB[H/64], S[16] (MA is read at
+// consecutive l, which needs no padding); sized per H so that small-H groups keep their LDS
+// footprint (and so the number of groups per CU) as low as the data allows.
 template<int H> constexpr int colTwPer() { return 64 + H / 64 + 16; }
-template<int H> constexpr int colTwSize() { return 64 + H / 64 + 2 * colTwPer<H>(); }
+template<int H> constexpr int colTwSize() { return TA_SLOTS + H / 64 + 2 * colTwPer<H>(); }
 template<int H>
 struct ColTw {
   static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
-  static constexpr int MID = 64 + H / 64;  // start of column 0's part
+  static constexpr int MID = TA_SLOTS + H / 64;  // start of column 0's part
   const T2* tab;
-  __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[m & 63], tab[64 + (m >> 6)]); }
+  __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
   __device__ __forceinline__ T2 mid(int h, int l) const {
     const T2* p = tab + MID + h * colTwPer<H>();
     return cmul(p[l & 63], p[64 + (l >> 6)]);
@@ -169,10 +184,11 @@ template<int H>
 __device__ __forceinline__ T2 colTwEntry(const JobDesc& J, int i, int a0, int a1) {
   constexpr int NHI = H / 64, MID = ColTw<H>::MID, PER = colTwPer<H>();
   T2 v;
-  if (i < 64) {
-    v = gld(J.twCol + i);
+  if (i < TA_SLOTS) {
+    const int e = taEntry(i);
+    v = e < 0 ? make_double2(0, 0) : gld(J.twCol + e);
   } else if (i < MID) {
-    v = gld(J.twCol + 64 * (i - 64));
+    v = gld(J.twCol + 64 * (i - TA_SLOTS));
   } else {
     const int h = (i - MID) / PER, r = (i - MID) % PER;
     const int a = h ? a1 : a0;
@@ -352,7 +368,9 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
         }
       }
       __syncthreads();
-      XStages<H, REG, 2, T, true, false, 1, C2, C1>::run(lds, tw, tid);
+      // inverse stages radix C1 then C2 (the reverse of the forward order would put the radix-C2
+      // stage first, whose stride-C2 writes hit few LDS banks for C2 = 15)
+      XStages<H, REG, 2, T, true, false, 1, C1, C2>::run(lds, tw, tid);
 
       // last inverse stage (radix C0, Ns = NB0), fused with the conjugate mid twiddle and the store
 #pragma unroll

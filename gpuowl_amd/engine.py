@@ -23,11 +23,20 @@ def kernel_names():
 
 
 def best_fit(E, spec=""):
-    """FFTConfig::bestFit: (spec, N, maxExp) of the FFT chosen for exponent E."""
+    """FFTConfig::bestFit: (spec, N, maxExp) of the FFT this engine chooses for exponent E."""
     lib = load()
     buf = C.create_string_buffer(64)
     N, mx = C.c_uint32(), C.c_uint32()
     check(lib.owl_fft_best_fit(E, (spec or "").encode(), buf, 64, C.byref(N), C.byref(mx)))
+    return buf.value.decode(), N.value, mx.value
+
+
+def best_fit_ref(E, spec=""):
+    """(spec, N, maxExp) of the FFT the reference chooses for exponent E (its limits, no tune file)."""
+    lib = load()
+    buf = C.create_string_buffer(64)
+    N, mx = C.c_uint32(), C.c_uint32()
+    check(lib.owl_fft_best_fit_ref(E, (spec or "").encode(), buf, 64, C.byref(N), C.byref(mx)))
     return buf.value.decode(), N.value, mx.value
 
 

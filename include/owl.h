@@ -44,8 +44,12 @@ typedef struct {
 } owl_ll_result_t;
 
 /* FFTConfig::bestFit (src/FFTConfig.cpp:235-263). spec may be "" or NULL: then the first FFT of the
- * loaded tune list whose maxExp covers E, else the first shape (variants 101, 202) that does. */
+ * loaded tune list whose maxExp covers E, else the first shape (variants 101, 202) that does.
+ * maxExp is this engine's: the reference's bits-per-word limit (src/fftbpw.h), lowered for the shapes
+ * where this engine's round-off at that limit is above the reference's criterion (Gumbel Z = 28). */
 int owl_fft_best_fit(uint32_t E, const char* spec, char* outSpec, size_t outLen, uint32_t* N, uint32_t* maxExp);
+/* The same selection with the reference's limits and no tune list: the FFT the reference picks. */
+int owl_fft_best_fit_ref(uint32_t E, const char* spec, char* outSpec, size_t outLen, uint32_t* N, uint32_t* maxExp);
 /* The reference's tune.txt (TuneEntry, src/TuneEntry.cpp:7-84): a process-wide list of FFTs ordered by
  * cost (microseconds per iteration) with maxExp increasing, read by bestFit when no spec is given
  * (owl_fft_best_fit, owl_gpu_create, batches, the server). Lines "cost spec [# maxExp]"; saved as

@@ -44,24 +44,47 @@ def test_no_fallback_when_library_missing(monkeypatch, tmp_path):
     (56_430_277, "1K:6:256:101", 3145728),
 ])
 def test_best_fit_matches_reference_choice(E, spec, N):
-    s, n, mx = gpuowl_amd.best_fit(E)
-    assert (s, n) == (spec, N)
-    assert mx >= E
+    for fit in (gpuowl_amd.best_fit_ref, gpuowl_amd.best_fit):  # the engine's limits agree here
+        s, n, mx = fit(E)
+        assert (s, n) == (spec, N)
+        assert mx >= E
 
 
 def test_best_fit_p139000003_max_exp():
-    s, n, mx = gpuowl_amd.best_fit(139_000_003)
-    assert abs(mx - 142_560_000) < 10_000
+    for fit in (gpuowl_amd.best_fit_ref, gpuowl_amd.best_fit):
+        s, n, mx = fit(139_000_003)
+        assert abs(mx - 142_560_000) < 10_000
 
 
 def test_best_fit_uneven200_distribution():
+    # the reference's choices for config D (its own limits)
     ps = workloads.uneven200()
     assert ps[:5] == [19_917_551, 11_330_491, 2_287_861, 3_503_053, 19_297_693]
     assert min(ps) == 1_029_689 and max(ps) == 56_430_277
-    Ns = [gpuowl_amd.best_fit(p)[1] for p in ps]
+    Ns = [gpuowl_amd.best_fit_ref(p)[1] for p in ps]
     assert Ns.count(262144) == 83 and Ns.count(393216) == 22
     assert sum(Ns) == 155_713_536
     assert sum(1 for p, n in zip(ps, Ns) if p / n < 12) == 58
+
+
+def test_engine_limits_cap_only_the_measured_shapes():
+    # The engine's limit is the reference's, lowered only for the shapes in engine_bpw.inc (their
+    # top exponents measured above the Z = 28 round-off criterion on this engine's kernels); of
+    # config D's 200 exponents, the five in those top ranges move to another FFT (of the same or the
+    # next size).
+    ps = workloads.uneven200()
+    moved = {p: (gpuowl_amd.best_fit_ref(p), gpuowl_amd.best_fit(p)) for p in ps}
+    moved = {p: (r, e) for p, (r, e) in moved.items() if r[:2] != e[:2]}
+    assert sorted(moved) == [5_124_299, 19_917_551, 24_741_121, 48_995_393, 49_039_027]
+    for p, (ref, eng) in moved.items():
+        assert eng[1] >= ref[1] and eng[0] != ref[0] and eng[2] >= p and ref[2] >= p
+        # the reference's FFT is still the choice with its limits, and p is above this engine's
+        assert gpuowl_amd.best_fit(p, ref[0])[2] < p
+    # the 256:2:256 cap: 19.442 bits per word (the reference's 101/202 limits are 19.547/19.636)
+    s, n, mx = gpuowl_amd.best_fit(5_000_000)
+    assert s == "256:2:256:101" and mx == int(19.442 * 262144)
+    assert gpuowl_amd.best_fit(5_110_000)[0] == "256:3:256:101"
+    assert gpuowl_amd.best_fit_ref(5_110_000)[0] == "256:2:256:101"
 
 
 def test_spec_parsing_and_errors():
