@@ -29,7 +29,9 @@ template<int W> constexpr int rowRegion() { return W + W / 16 + 4; }
 template<int H> constexpr int colRegion() { return H + H / 16 + 8; }
 
 // One LDS -> LDS radix-R autosort stage over NFFT FFTs of length L stored in
-// padded regions; Ns is the product of the earlier radices. tw[m] = exp(-2 pi i m/L).
+// padded regions; Ns is the product of the earlier radices. The twiddle of butterfly input k
+// at p = j mod Ns is exp(-2 pi i p*k/(Ns*R)) = tw(p*STRIDE*k) with tw(m) = exp(-2 pi i m/L); the
+// provider's st<Ns*R>(p, k, STRIDE) returns it (from a table of the (Ns*R)-th roots when it has one).
 // PRIV: the caller allows wave-private stages. A stage with 64 butterflies per FFT
 // (or a one-wave group) reads and writes only its own wave's regions; LDS operations of
 // one wave complete in order, so such a stage needs no workgroup barrier, only a compiler
@@ -63,7 +65,7 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
         const int p = j % Ns;
 #pragma unroll
         for (int k = 1; k < R; ++k) {
-          T2 w = tw(p * STRIDE * k);
+          T2 w = tw.template st<Ns * R>(p, k, STRIDE);
           v[m][k] = INV ? cmulc(v[m][k], w) : cmul(v[m][k], w);
         }
       }
@@ -125,7 +127,18 @@ struct XStages<L, REG, NFFT, T, INV, PRIV, Ns, R, Rs...> {
 struct GlobalTw {
   const T2* t;
   __device__ __forceinline__ T2 operator()(int m) const { return gld(t + m); }
+  template<int NR> __device__ __forceinline__ T2 st(int p, int k, int stride) const { return (*this)(p * stride * k); }
 };
+
+// Exact tables of the n-th roots w_n^q (q < n, n = 256 or n = H/16 in the column pass) for the
+// stages whose twiddles are exactly those roots (a stage with Ns*R = n reads w_n^((p*k) mod n)):
+// one correctly rounded entry per twiddle instead of the product of two. Stored like TA with
+// one pad slot per 16 entries (entry q at q + q/16).
+template<int n> constexpr int rootSlots() { return n + n / 16; }
+template<int n>
+__device__ __forceinline__ void fillRoots(const T2* full, int stride, T2* tab, int tid, int T) {
+  for (int q = tid; q < n; q += T) { tab[q + (q >> 4)] = gld(full + q * stride); }
+}
 
 // The low-part tables TA[i] = w^i (i < 64) below are read at i = (p*k*stride) & 63 for the lanes
 // p of a wave and a fixed k; for even k many lanes of one LDS access phase would then hit the same
@@ -142,20 +155,34 @@ __device__ __forceinline__ constexpr int taEntry(int p) { return (p % 17 == 16) 
 // round-off slots): w_W^m = TA[m & 63] * TB[m >> 6], TA[i] = w_W^i, TB[j] = w_W^(64 j),
 // each twiddle a product of two correctly rounded table entries. Used for W >= 512; at
 // W = 256 the fill (68 entries for a 16 KB block) costs more than the global reads save.
+// The stages whose twiddles are NROOT-th roots or coarser (Ns*R dividing NROOT = W/F0: the inverse
+// stage before the last and the second forward stage) read them from an exact table of the
+// NROOT-th roots after it (rootSlots<NROOT>() entries).
 constexpr int ROWTW_SIZE = TA_SLOTS + 64;
+template<int NROOT> constexpr int rowTwSlots() { return ROWTW_SIZE + rootSlots<NROOT>(); }
+template<int NROOT>
 struct RowTw {
   const T2* tab;
   __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
+  template<int NR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
+    if constexpr (NROOT % NR == 0) {
+      const int q = (p * k * (NROOT / NR)) % NROOT;
+      return tab[ROWTW_SIZE + q + (q >> 4)];
+    } else {
+      return (*this)(p * stride * k);
+    }
+  }
 };
 template<int W, int T>
 __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
-  static_assert(W % 64 == 0 && W / 64 <= 64, "twiddle table");
+  static_assert(W % 64 == 0 && W / 64 <= 64 && W % T == 0, "twiddle table");
   constexpr int N = TA_SLOTS + W / 64;
 #pragma unroll
   for (int p = tid; p < N; p += T) {
     const int e = p < TA_SLOTS ? taEntry(p) : 64 * (p - TA_SLOTS);
     tab[p] = e < 0 ? make_double2(0, 0) : gld(twRow + e);
   }
+  fillRoots<T>(twRow, W / T, tab + ROWTW_SIZE, tid, T);
 }
 
 // Column-pass twiddles staged in LDS after the data regions (colTwSize<H>() entries):
@@ -167,12 +194,28 @@ __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
 // footprint (and so the number of groups per CU) as low as the data allows.
 template<int H> constexpr int colTwPer() { return 64 + H / 64 + 16; }
 template<int H> constexpr int colTwSize() { return TA_SLOTS + H / 64 + 2 * colTwPer<H>(); }
-template<int H>
+// With ROOTS, the stages with Ns*R dividing 256 (the second forward stage: Ns = 16, R = C1) and
+// Ns*R = H/16 (the second inverse stage: Ns = C1, R = C2) read exact root tables (roots, after the
+// pair tables: rootSlots<256>() entries, then rootSlots<H/16>() unless H/16 divides 256).
+template<int H> constexpr int colRootSlots() { return rootSlots<256>() + (256 % (H / 16) == 0 ? 0 : rootSlots<H / 16>()); }
+template<int H, bool ROOTS = false>
 struct ColTw {
   static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
   static constexpr int MID = TA_SLOTS + H / 64;  // start of column 0's part
   const T2* tab;
+  const T2* roots = nullptr;
   __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
+  template<int NR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
+    if constexpr (ROOTS && 256 % NR == 0) {
+      const int q = (p * k * (256 / NR)) & 255;
+      return roots[q + (q >> 4)];
+    } else if constexpr (ROOTS && NR == H / 16) {
+      const int q = (p * k) % NR;
+      return roots[rootSlots<256>() + q + (q >> 4)];
+    } else {
+      return (*this)(p * stride * k);
+    }
+  }
   __device__ __forceinline__ T2 mid(int h, int l) const {
     const T2* p = tab + MID + h * colTwPer<H>();
     return cmul(p[l & 63], p[64 + (l >> 6)]);
@@ -206,12 +249,18 @@ template<int H>
 __device__ __forceinline__ void fillColTw(const JobDesc& J, T2* tab, int a0, int a1, int tid, int T) {
   for (int i = tid; i < colTwSize<H>(); i += T) { tab[i] = colTwEntry<H>(J, i, a0, a1); }
 }
+template<int H>
+__device__ __forceinline__ void fillColRoots(const JobDesc& J, T2* roots, int tid, int T) {
+  fillRoots<256>(J.twCol, H / 256, roots, tid, T);
+  if constexpr (256 % (H / 16) != 0) { fillRoots<H / 16>(J.twCol, 16, roots + rootSlots<256>(), tid, T); }
+}
 
 // ---------------------------------------------------------------- column pass
 // MULTI = false: grid (W/2, batch), group q handles the column pair q; LDS: two
 // colRegion<H>() regions and one twiddle table (colTwSize<H>() entries).
 // MULTI = true: grid (G, batch) with G <= W/2, group x handles the pairs x, x + G, x + 2G,
 // ... below W/2; LDS: the two regions and two tables. T threads; modes as k_col.
+// For H >= 2304 the root tables (colRootSlots<H>() entries) follow the twiddle table(s).
 // With several pairs per group, the next pair's column data and twiddle-table entries
 // are requested right after the current pair's first stage and stay in registers while
 // the current pair is transformed, squared and stored, so that HBM reads overlap the
@@ -228,6 +277,9 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
   constexpr int MB0 = (NBT + T - 1) / T;
   constexpr int TWS = colTwSize<H>();
   constexpr int MT = (TWS + T - 1) / T;
+  // exact root tables for two of the stages where LDS allows (one group per CU anyway)
+  constexpr bool ROOTS = H >= 2304;
+  T2* const roots = lds + 2 * REG + (MULTI ? 2 : 1) * TWS;
   const JobDesc& J = jobs[blockIdx.y];
   const int W = J.W, R = J.R, lgR = J.lgR;
   const int tid0 = threadIdx.x;
@@ -267,13 +319,14 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
   // the column data is requested before the twiddle table is filled, so the two latencies overlap
   loadPair(q, tid0);
   fillColTw<H>(J, lds + 2 * REG, q, q ? W - q : W / 2, tid0, T);
+  if constexpr (ROOTS) { fillColRoots<H>(J, roots, tid0, T); }
   __syncthreads();
   for (int it = 0;; ++it) {
     // a fresh copy of the thread index per pair: offsets derived from it are recomputed
     // in each pair instead of being hoisted out of the loop and held in registers
     int tid = tid0;
     if constexpr (MULTI) { asm volatile("" : "+v"(tid)); }
-    const ColTw<H> tw{lds + 2 * REG + (it & 1) * TWS};
+    const ColTw<H, ROOTS> tw{lds + 2 * REG + (it & 1) * TWS, roots};
     const int a0 = q, a1 = q ? W - q : W / 2;
     const size_t colBase = (size_t)2 * q * R;
     const T2 wa = gld(J.twPair + (q ? a0 : a1));  // column a0's pairing twiddle, or column W/2's for q = 0
@@ -601,7 +654,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   static constexpr bool LDS_TW = W >= 512;
   // the block streams through with non-temporal accesses from W = 512 up
   static constexpr bool NT = W >= 512;
-  using Tw = std::conditional_t<LDS_TW, RowTw, GlobalTw>;
+  using Tw = std::conditional_t<LDS_TW, RowTw<T>, GlobalTw>;
 
   // element e = s*R + r of the block (slot s, line r) for e = tid + i*T
   __device__ __forceinline__ static void load(const T2* blk, T2* P, int tid) {
