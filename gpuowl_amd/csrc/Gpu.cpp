@@ -218,6 +218,31 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   J.twMidS = trig_->midS.get();
   J.lineW = upload(lineW);
   J.colW = upload(colW);
+  J.rowMeta = nullptr;
+  if (rowFixed) {
+    // Per row-pass thread and block: which of its words are big and which weight exponents wrap
+    // (the tests of wordWeight, src/Gpu.cpp:40-127's bitsCF), so the pass does not recompute them.
+    const u32 T = rowT, F0 = W / T;
+    std::vector<u64> meta(size_t(nb) * T, 0);
+    for (u32 lb = 0; lb < nb; ++lb) {
+      for (u32 tid = 0; tid < T; ++tid) {
+        u64 m = 0;
+        for (u32 k = 0; k < F0; ++k) {
+          for (u32 r = 0; r < R; ++r) {
+            for (u32 rep = 0; rep < 2; ++rep) {
+              const u32 w = (k * R + r) * 2 + rep;
+              const u32 t = lineW[lb * R + r].e[rep] + colW[tid + k * T].e;
+              const bool wrap = t >= N;
+              const bool big = (wrap ? t - N : t) < J.rE;
+              m |= (u64(big) << w) | (u64(wrap) << (32 + w));
+            }
+          }
+        }
+        meta[size_t(lb) * T + tid] = m;
+      }
+    }
+    J.rowMeta = upload(meta);
+  }
   for (int i = 0; i < 3; ++i) { J.freq[i] = alloc<double2>(ND); }
   for (int i = 0; i < 4; ++i) { J.words[i] = alloc<int2>(ND); }
   for (int i = 0; i < 2; ++i) { J.carry[i] = alloc<int64_t>(size_t(nb) * W); }
@@ -331,9 +356,10 @@ u64 Gpu::deviceBytes() const {
 u64 Gpu::footprintBound(u32 N, u32 W, u32 H) {
   const u64 ND = N / 2;
   // 3 spectra (16 B) + 4 word buffers (8 B) per complex point; carries and stairs
-  // (3 x 8 B per column and block, at most ND); the carry fix map (one bit per column and
-  // block, plus a bit per map word); weights; round-off, flags, residue
-  return ND * (3 * 16 + 4 * 8) + 3 * 8 * ND + ND / 8 + ND / 256 + 16 + u64(H) * sizeof(LineW) + u64(W) * sizeof(ColW) +
+  // (3 x 8 B per column and block, at most ND); the row pass's word metadata (8 B per thread and
+  // block, at most ND bytes); the carry fix map (one bit per column and block, plus a bit per map
+  // word); weights; round-off, flags, residue
+  return ND * (3 * 16 + 4 * 8) + 3 * 8 * ND + ND + ND / 8 + ND / 256 + 16 + u64(H) * sizeof(LineW) + u64(W) * sizeof(ColW) +
          4 * OWL_ROE_SIZE + 4096;
 }
 

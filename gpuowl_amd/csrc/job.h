@@ -70,6 +70,9 @@ struct JobDesc {
   uint64_t* ticket;       // fused row-pass start counter: every launch takes nb+1 tickets
   uint32_t* roe;          // OWL_ROE_SIZE max-round-off slots (float bits)
   uint32_t* flags;        // [0] carry leftover, [1] compare mismatch
+  uint64_t* rowMeta;      // shape-specialized row pass: nb*T words (T = W/F0 threads), word (block, tid):
+                          // bit w = big word, bit 32+w = weight exponent wrapped, for the thread's
+                          // words w = (k*R + r)*2 + rep (column tid + k*T, line block*R + r)
   uint64_t* fixMap;       // ceil(nb*W/64) words: bit rank set where a carry left its block, then
                           // ceil(nb*W/4096) summary words: bit w set where map word w is nonzero
   int32_t* small;         // 64 ints for the residue read

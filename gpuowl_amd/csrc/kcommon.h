@@ -52,6 +52,7 @@ __device__ __forceinline__ void gst(int2* p, int2 x) {
   *(GLOBAL v2i*)p = v;
 }
 __device__ __forceinline__ int64_t gld(const int64_t* p) { return *(const GLOBAL int64_t*)p; }
+__device__ __forceinline__ uint64_t gld(const uint64_t* p) { return *(const GLOBAL uint64_t*)p; }
 __device__ __forceinline__ void gst(int64_t* p, int64_t x) { *(GLOBAL int64_t*)p = x; }
 __device__ __forceinline__ LineW gld(const LineW* p) {
   const GLOBAL LineW* g = (const GLOBAL LineW*)p;

@@ -605,14 +605,6 @@ __device__ __forceinline__ LineWs lineWs(const LineW* p) {
   const CONSTANT LineW* q = (const CONSTANT LineW*)p;
   return {q->w[0], q->w[1], q->iw[0], q->iw[1], q->e[0], q->e[1]};
 }
-// Bits of word (line, column, rep) and whether its weight exponent wrapped (see wordWeight).
-__device__ __forceinline__ int wordBits(const JobDesc& J, uint32_t le, uint32_t ce, bool& wrap) {
-  uint32_t e = le + ce;
-  wrap = e >= J.N;
-  uint32_t ex = wrap ? e - J.N : e;
-  return J.bitsLo + (ex < J.rE);
-}
-
 // Tagged stairway access through the global address space.
 __device__ __forceinline__ uint64_t stairLoad(const uint64_t* p) {
   return __hip_atomic_load((const GLOBAL uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -662,13 +654,38 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
     for (int i = 0; i < NL; ++i) { P[i] = ldx<NT>(blk + tid + i * T); }
   }
   // ... into the line regions in frequency order
-  __device__ __forceinline__ static void scatter(T2* lds, const T2* P, int tid) {
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int e = tid + i * T;
-      const int s = e / R, r = e % R;
-      lds[r * REG + padi(slotFreq(s, W))] = P[i];
+  // LDS position of the block's element e = tid + i*T (slot s = e/R, line r = e%R) in the line
+  // regions, frequency order. When T/(2R) is a multiple of 16 the slot of element i is
+  // s0 + i*T/R with the same parity as s0, so its frequency moves by +-i*T/(2R) from s0's and its
+  // padded position by +-i*(T/(2R))*17/16: one multiply-add per element instead of the division
+  // and the slot-to-frequency mapping (the one exception, frequency W/2 in slot 1, is i = 0).
+  static constexpr bool LINEAR_SLOTS = T % (2 * R) == 0 && (T / (2 * R)) % 16 == 0;
+  struct SlotPos {
+    int base, delta, first;
+    __device__ __forceinline__ SlotPos(int tid) {
+      if constexpr (LINEAR_SLOTS) {
+        const int s0 = tid / R, r = tid % R, q0 = s0 >> 1;
+        const bool odd = s0 & 1;
+        base = r * REG + (odd ? padi(W - q0) : padi(q0));
+        delta = (odd ? -1 : 1) * (T / (2 * R)) * 17 / 16;
+        first = r * REG + padi(slotFreq(s0, W));
+      } else {
+        base = delta = first = 0;
+      }
     }
+    __device__ __forceinline__ int operator()(int tid, int i) const {
+      if constexpr (LINEAR_SLOTS) {
+        return i == 0 ? first : base + i * delta;
+      } else {
+        const int e = tid + i * T;
+        return (e % R) * REG + padi(slotFreq(e / R, W));
+      }
+    }
+  };
+  __device__ __forceinline__ static void scatter(T2* lds, const T2* P, int tid) {
+    const SlotPos pos(tid);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) { lds[pos(tid, i)] = P[i]; }
   }
   // inverse W-point FFT of the R lines (LDS stages, then the radix-F0 stage into registers)
   __device__ __forceinline__ static void inverse(T2* lds, const Tw& tw, int tid, T2 (&v)[R][F0]) {
@@ -689,11 +706,22 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   // and floor are exact. Word u of b bits splits into the carry floor(u/2^b + 1/2) and the
   // balanced digit u - carry*2^b in [-2^(b-1), 2^(b-1)), as the integer chain would.
   // cs[k] = the block's carry-out of column k; roe = max round-off.
-  __device__ __forceinline__ static void carry(const JobDesc& J, const LineW* lwBase, const ColW* cw, T2 (&v)[R][F0],
-                                               double* cs, double& roe, bool llHere, bool mul3) {
+  // Word metadata (J.rowMeta, one 64-bit word per thread and block): for the thread's word
+  // w = (k*R + r)*2 + rep, bit w says whether it is big (bitsLo + 1 bits) and bit 32 + w whether
+  // its weight exponent wrapped (weight halved, inverse weight doubled).
+  static_assert(F0 * R * 2 <= 32, "row metadata");
+  __device__ __forceinline__ static int bigBit(uint64_t meta, int k, int r, int rep) {
+    return (int)((uint32_t)meta >> ((k * R + r) * 2 + rep)) & 1;
+  }
+  __device__ __forceinline__ static int wrapBit(uint64_t meta, int k, int r, int rep) {
+    return (int)((uint32_t)(meta >> 32) >> ((k * R + r) * 2 + rep)) & 1;
+  }
+  __device__ __forceinline__ static void carry(const JobDesc& J, const LineW* lwBase, const ColW* cw, uint64_t meta,
+                                               T2 (&v)[R][F0], double* cs, double& roe, bool llHere, bool mul3) {
 #pragma unroll
     for (int k = 0; k < F0; ++k) { cs[k] = 0; }
     const double m3 = mul3 ? 3.0 : 1.0;
+    const int bitsLo = J.bitsLo;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const LineWs lw = lineWs(lwBase + r);
@@ -701,9 +729,8 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
       for (int k = 0; k < F0; ++k) {
 #pragma unroll
         for (int rep = 0; rep < 2; ++rep) {
-          bool wrap;
-          const int b = wordBits(J, rep ? lw.e1 : lw.e0, cw[k].e, wrap);
-          const double iw = __builtin_amdgcn_ldexp((rep ? lw.iw1 : lw.iw0) * cw[k].iw, wrap ? 1 : 0);
+          const int b = bitsLo + bigBit(meta, k, r, rep);
+          const double iw = __builtin_amdgcn_ldexp((rep ? lw.iw1 : lw.iw0) * cw[k].iw, wrapBit(meta, k, r, rep));
           const double x = rep ? v[r][k].y : v[r][k].x;
           const double t = fma(x, iw, OWL_RNDVAL) - OWL_RNDVAL;
           roe = fmax(roe, fabs(fma(x, iw, -t)));
@@ -722,17 +749,15 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
     }
   }
   // Ripples the predecessor's carries through each column's 2R words until they die.
-  __device__ __forceinline__ static void ripple(const JobDesc& J, const LineW* lwBase, const ColW* cw, T2 (&v)[R][F0],
-                                                const uint64_t* got) {
+  __device__ __forceinline__ static void ripple(const JobDesc& J, uint64_t meta, T2 (&v)[R][F0], const uint64_t* got) {
+    const int bitsLo = J.bitsLo;
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
       double carry = (double)unpackCarry(got[k]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (carry != 0) {
-          const LineWs lw = lineWs(lwBase + r);
-          bool wrap;
-          const int b0 = wordBits(J, lw.e0, cw[k].e, wrap);
+          const int b0 = bitsLo + bigBit(meta, k, r, 0);
           double t = v[r][k].x + carry;
           carry = floor(__builtin_amdgcn_ldexp(t, -b0) + 0.5);
           v[r][k].x = t - __builtin_amdgcn_ldexp(carry, b0);
@@ -740,7 +765,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
             v[r][k].y += carry;
             carry = 0;
           } else if (carry != 0) {
-            const int b1 = wordBits(J, lw.e1, cw[k].e, wrap);
+            const int b1 = bitsLo + bigBit(meta, k, r, 1);
             t = v[r][k].y + carry;
             carry = floor(__builtin_amdgcn_ldexp(t, -b1) + 0.5);
             v[r][k].y = t - __builtin_amdgcn_ldexp(carry, b1);
@@ -750,18 +775,14 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
     }
   }
   // Weight, then the forward W-point FFT's stage 0 (radix F0, Ns = 1), in registers.
-  __device__ __forceinline__ static void weightStage0(const JobDesc& J, const LineW* lwBase, const ColW* cw,
-                                                      T2 (&v)[R][F0]) {
+  __device__ __forceinline__ static void weightStage0(const LineW* lwBase, const ColW* cw, uint64_t meta, T2 (&v)[R][F0]) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const LineWs lw = lineWs(lwBase + r);
 #pragma unroll
       for (int k = 0; k < F0; ++k) {
-        bool wrap0, wrap1;
-        wordBits(J, lw.e0, cw[k].e, wrap0);
-        wordBits(J, lw.e1, cw[k].e, wrap1);
-        const double w0 = __builtin_amdgcn_ldexp(lw.w0 * cw[k].w, wrap0 ? -1 : 0);
-        const double w1 = __builtin_amdgcn_ldexp(lw.w1 * cw[k].w, wrap1 ? -1 : 0);
+        const double w0 = __builtin_amdgcn_ldexp(lw.w0 * cw[k].w, -wrapBit(meta, k, r, 0));
+        const double w1 = __builtin_amdgcn_ldexp(lw.w1 * cw[k].w, -wrapBit(meta, k, r, 1));
         v[r][k] = make_double2(v[r][k].x * w0, v[r][k].y * w1);
       }
       Dft<F0, false>::run(v[r]);
@@ -781,12 +802,9 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   }
   // the block from the line regions, slot order
   __device__ __forceinline__ static void store(T2* blk, const T2* lds, int tid) {
+    const SlotPos pos(tid);
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int e = tid + i * T;
-      const int s = e / R, r = e % R;
-      stx<NT>(blk + e, lds[r * REG + padi(slotFreq(s, W))]);
-    }
+    for (int i = 0; i < NL; ++i) { stx<NT>(blk + tid + i * T, lds[pos(tid, i)]); }
   }
   // the stairway carries this block (g) waits for: block g-1's, shifted by one column for the
   // redo of block 0 (g == nb, the wrap-around)
@@ -840,6 +858,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   ColW cw[F0];
 #pragma unroll
   for (int k = 0; k < F0; ++k) { cw[k] = gld(J.colW + tid + k * T); }
+  const uint64_t meta = gld(J.rowMeta + (size_t)lb * T + tid);
   const LineW* lwBase = J.lineW + lb * R;
 
   if (mode == ROW_LEADIN) {
@@ -862,7 +881,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
     P::inverse(lds, tw, tid, v);
     double roe = 0;
     double cs[F0];
-    P::carry(J, lwBase, cw, v, cs, roe, ll && lb == 0 && tid == 0, mul3);
+    P::carry(J, lwBase, cw, meta, v, cs, roe, ll && lb == 0 && tid == 0, mul3);
     if (mode == ROW_CARRYOUT) {
       roeReport(J, roeSlot, (float)roe);
       int2* dst = J.words[wi] + (size_t)lb * LW;
@@ -883,9 +902,9 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
     roeWave(roeSlot, (float)roe, roePart);
     uint64_t got[F0];
     P::pollPredecessor(J, g, tid, tag, got);
-    P::ripple(J, lwBase, cw, v, got);
+    P::ripple(J, meta, v, got);
   }
-  P::weightStage0(J, lwBase, cw, v);
+  P::weightStage0(lwBase, cw, meta, v);
   __syncthreads();  // every thread has finished reading the inverse's LDS
   if (mode == ROW_FUSED) { roeGroup(J, roeSlot, roePart, T / 64); }
   P::forwardLds(v, lds, tw, tid);
