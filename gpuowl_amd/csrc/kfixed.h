@@ -65,7 +65,7 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
         const int p = j % Ns;
 #pragma unroll
         for (int k = 1; k < R; ++k) {
-          T2 w = tw.template st<Ns * R>(p, k, STRIDE);
+          T2 w = tw.template st<Ns * R, R>(p, k, STRIDE);
           v[m][k] = INV ? cmulc(v[m][k], w) : cmul(v[m][k], w);
         }
       }
@@ -127,17 +127,23 @@ struct XStages<L, REG, NFFT, T, INV, PRIV, Ns, R, Rs...> {
 struct GlobalTw {
   const T2* t;
   __device__ __forceinline__ T2 operator()(int m) const { return gld(t + m); }
-  template<int NR> __device__ __forceinline__ T2 st(int p, int k, int stride) const { return (*this)(p * stride * k); }
+  template<int NR, int RR> __device__ __forceinline__ T2 st(int p, int k, int stride) const { return (*this)(p * stride * k); }
 };
 
-// Exact tables of the n-th roots w_n^q (q < n, n = 256 or n = H/16 in the column pass) for the
-// stages whose twiddles are exactly those roots (a stage with Ns*R = n reads w_n^((p*k) mod n)):
-// one correctly rounded entry per twiddle instead of the product of two. Stored like TA with
-// one pad slot per 16 entries (entry q at q + q/16).
-template<int n> constexpr int rootSlots() { return n + n / 16; }
-template<int n>
-__device__ __forceinline__ void fillRoots(const T2* full, int stride, T2* tab, int tid, int T) {
-  for (int q = tid; q < n; q += T) { tab[q + (q >> 4)] = gld(full + q * stride); }
+// Exact product tables for the stages whose twiddles are n-th roots with n = Ns*R small (Ns <= 16
+// butterfly phases p, R <= 16): entry (p, k) = w_n^(p*k) for p < P and 1 <= k <= K, one correctly
+// rounded value per twiddle instead of the product of two, stored at p*KS + k - 1 with an odd row
+// stride KS >= K. The 16 lanes of one LDS access phase read (p, k) for 16 consecutive p and one k:
+// 16 entries KS apart, which an odd KS puts in 16 distinct 16-byte bank slots (64 banks). (A table
+// of the n-th roots indexed by p*k puts them 2 to 16 in one slot for most k, padded or not.)
+// `full` holds w_L^m at m, and w_n = w_L^stride.
+template<int P, int K, int KS>
+__device__ __forceinline__ void fillProd(const T2* full, int stride, T2* tab, int tid, int T) {
+  static_assert(KS % 2 == 1 && KS >= K, "product table stride");
+  for (int i = tid; i < P * KS; i += T) {
+    const int p = i / KS, k = i % KS + 1;
+    tab[i] = k <= K ? gld(full + p * k * stride) : make_double2(0, 0);
+  }
 }
 
 // The low-part tables TA[i] = w^i (i < 64) below are read at i = (p*k*stride) & 63 for the lanes
@@ -155,25 +161,26 @@ __device__ __forceinline__ constexpr int taEntry(int p) { return (p % 17 == 16) 
 // round-off slots): w_W^m = TA[m & 63] * TB[m >> 6], TA[i] = w_W^i, TB[j] = w_W^(64 j),
 // each twiddle a product of two correctly rounded table entries. Used for W >= 512; at
 // W = 256 the fill (68 entries for a 16 KB block) costs more than the global reads save.
-// The stages whose twiddles are NROOT-th roots or coarser (Ns*R dividing NROOT = W/F0: the inverse
-// stage before the last and the second forward stage) read them from an exact table of the
-// NROOT-th roots after it (rootSlots<NROOT>() entries).
+// The radix-F1 stages whose twiddles are (F1*F2)-th roots or coarser (the inverse stage before the
+// last, Ns = F2, and the second forward stage, Ns = F0 dividing F2) read them from the product
+// table of the (F1*F2)-th roots after it (fillProd: F2 rows, stride F1 - 1; at most T = W/F0 =
+// F1*F2 slots are reserved): phase p of such a stage reads row p * (F1*F2) / (Ns*F1).
 constexpr int ROWTW_SIZE = TA_SLOTS + 64;
-template<int NROOT> constexpr int rowTwSlots() { return ROWTW_SIZE + rootSlots<NROOT>(); }
-template<int NROOT>
+template<int T> constexpr int rowTwSlots() { return ROWTW_SIZE + T; }
+template<int F1, int F2>
 struct RowTw {
+  static_assert(F1 % 2 == 0 && F2 * (F1 - 1) <= F1 * F2, "row product table");
   const T2* tab;
   __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
-  template<int NR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
-    if constexpr (NROOT % NR == 0) {
-      const int q = (p * k * (NROOT / NR)) % NROOT;
-      return tab[ROWTW_SIZE + q + (q >> 4)];
+  template<int NR, int RR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
+    if constexpr (RR == F1 && (F1 * F2) % NR == 0) {
+      return tab[ROWTW_SIZE + p * ((F1 * F2) / NR) * (F1 - 1) + k - 1];
     } else {
       return (*this)(p * stride * k);
     }
   }
 };
-template<int W, int T>
+template<int W, int T, int F1, int F2>
 __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
   static_assert(W % 64 == 0 && W / 64 <= 64 && W % T == 0, "twiddle table");
   constexpr int N = TA_SLOTS + W / 64;
@@ -182,7 +189,7 @@ __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
     const int e = p < TA_SLOTS ? taEntry(p) : 64 * (p - TA_SLOTS);
     tab[p] = e < 0 ? make_double2(0, 0) : gld(twRow + e);
   }
-  fillRoots<T>(twRow, W / T, tab + ROWTW_SIZE, tid, T);
+  fillProd<F2, F1 - 1, F1 - 1>(twRow, W / (F1 * F2), tab + ROWTW_SIZE, tid, T);
 }
 
 // Column-pass twiddles staged in LDS after the data regions (colTwSize<H>() entries):
@@ -194,10 +201,12 @@ __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
 // footprint (and so the number of groups per CU) as low as the data allows.
 template<int H> constexpr int colTwPer() { return 64 + H / 64 + 16; }
 template<int H> constexpr int colTwSize() { return TA_SLOTS + H / 64 + 2 * colTwPer<H>(); }
-// With ROOTS, the stages with Ns*R dividing 256 (the second forward stage: Ns = 16, R = C1) and
-// Ns*R = H/16 (the second inverse stage: Ns = C1, R = C2) read exact root tables (roots, after the
-// pair tables: rootSlots<256>() entries, then rootSlots<H/16>() unless H/16 divides 256).
-template<int H> constexpr int colRootSlots() { return rootSlots<256>() + (256 % (H / 16) == 0 ? 0 : rootSlots<H / 16>()); }
+// With ROOTS (H >= 2304, plan [16, 16, H/256]), the second forward stage (Ns = 16, R = 16: 256th
+// roots) and the second inverse stage (Ns = 16, R = H/256: (H/16)-th roots) read exact product
+// tables (fillProd; roots, after the pair tables): A, 16 rows of stride 15, then B, 16 rows of stride
+// (H/256 - 1) | 1, unless H/16 = 256 and A serves both.
+template<int H> constexpr int colProdStrideB() { return (H / 256 - 1) | 1; }
+template<int H> constexpr int colRootSlots() { return 16 * 15 + (H == 4096 ? 0 : 16 * colProdStrideB<H>()); }
 template<int H, bool ROOTS = false>
 struct ColTw {
   static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
@@ -205,13 +214,11 @@ struct ColTw {
   const T2* tab;
   const T2* roots = nullptr;
   __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
-  template<int NR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
-    if constexpr (ROOTS && 256 % NR == 0) {
-      const int q = (p * k * (256 / NR)) & 255;
-      return roots[q + (q >> 4)];
-    } else if constexpr (ROOTS && NR == H / 16) {
-      const int q = (p * k) % NR;
-      return roots[rootSlots<256>() + q + (q >> 4)];
+  template<int NR, int RR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
+    if constexpr (ROOTS && NR == 256 && RR == 16) {
+      return roots[p * 15 + k - 1];
+    } else if constexpr (ROOTS && NR == H / 16 && RR == H / 256) {
+      return roots[16 * 15 + p * colProdStrideB<H>() + k - 1];
     } else {
       return (*this)(p * stride * k);
     }
@@ -251,8 +258,9 @@ __device__ __forceinline__ void fillColTw(const JobDesc& J, T2* tab, int a0, int
 }
 template<int H>
 __device__ __forceinline__ void fillColRoots(const JobDesc& J, T2* roots, int tid, int T) {
-  fillRoots<256>(J.twCol, H / 256, roots, tid, T);
-  if constexpr (256 % (H / 16) != 0) { fillRoots<H / 16>(J.twCol, 16, roots + rootSlots<256>(), tid, T); }
+  static_assert(H % 256 == 0 && H / 256 <= 16, "column product tables");
+  fillProd<16, 15, 15>(J.twCol, H / 256, roots, tid, T);
+  if constexpr (H != 4096) { fillProd<16, H / 256 - 1, colProdStrideB<H>()>(J.twCol, 16, roots + 16 * 15, tid, T); }
 }
 
 // ---------------------------------------------------------------- column pass
@@ -646,7 +654,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   static constexpr bool LDS_TW = W >= 512;
   // the block streams through with non-temporal accesses from W = 512 up
   static constexpr bool NT = W >= 512;
-  using Tw = std::conditional_t<LDS_TW, RowTw<T>, GlobalTw>;
+  using Tw = std::conditional_t<LDS_TW, RowTw<F1, F2>, GlobalTw>;
 
   // element e = s*R + r of the block (slot s, line r) for e = tid + i*T
   __device__ __forceinline__ static void load(const T2* blk, T2* P, int tid) {
@@ -831,7 +839,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
 };
 
 // Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 + 64
-// (+ ROWTW_SIZE*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle table.
+// (+ rowTwSlots<T>()*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle tables.
 // Modes, options and the tagged stairway as k_row.
 template<int W, int R, int F0, int F1, int F2>
 __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts,
@@ -852,7 +860,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   const int tid = threadIdx.x;
   const bool ll = opts & 1, mul3 = opts & 2;
   T2* blk = J.freq[fi] + (size_t)lb * LW;
-  if constexpr (P::LDS_TW) { fillRowTw<W, T>(J.twRow, twTab, tid); }  // read after the first barrier below
+  if constexpr (P::LDS_TW) { fillRowTw<W, T, F1, F2>(J.twRow, twTab, tid); }  // read after the first barrier below
 
   T2 v[R][F0];
   ColW cw[F0];
