@@ -141,9 +141,9 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &rowT), "kernel");
-    // R regions of rowRegion<W>() (kfixed.h) + per-wave round-off + the rowTwSlots<rowT>()-entry
-    // twiddle and root table (W >= 512)
-    rowLds = R * (W + W / 16 + 4) * 16 + 64 + (W >= 512 ? (132 + rowT) * 16 : 0);
+    // R regions of rowRegion<W>() (kfixed.h) + per-wave round-off + the rowTwSlots<W, rowT>()-entry
+    // twiddle tables (W >= 512)
+    rowLds = R * (W + W / 16 + 4) * 16 + 64 + (W >= 1024 ? (rowT + 240 + W / 16) * 16 : W >= 512 ? (132 + rowT) * 16 : 0);
   } else {
     rowKernel = "row" + std::to_string(rowT);
   }
@@ -154,7 +154,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     // colRegion<H>() regions + a colTwSize<H>()-entry twiddle table (kfixed.h)
     const u32 twSize = 68 + H / 64 + 2 * (64 + H / 64 + 16);
     // the two-region pass adds exact root tables for H >= 2304 (colRootSlots<H>())
-    const u32 rootSlots = (!useY && H >= 2304) ? 16 * 15 + (H == 4096 ? 0 : 16 * ((H / 256 - 1) | 1)) : 0;
+    const u32 rootSlots = (!useY && H >= 2304) ? 16 * 15 + (H == 4096 ? 0 : 16 * ((H / 256 - 1) | 1) + 240 + H / 16) : 0;
     colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + twSize + rootSlots) * 16;
     if (useX && !opt.colOnePair && haveKernel(colX + "m")) {
       int cus = 0, lds = 0;

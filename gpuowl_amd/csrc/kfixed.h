@@ -61,7 +61,7 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
 #pragma unroll
         for (int k = 0; k < R; ++k) { v[m][k] = x[padi(j + k * NB)]; }
       }
-      if (Ns > 1) {
+      if constexpr (Ns > 1) {
         const int p = j % Ns;
 #pragma unroll
         for (int k = 1; k < R; ++k) {
@@ -128,6 +128,7 @@ struct GlobalTw {
   const T2* t;
   __device__ __forceinline__ T2 operator()(int m) const { return gld(t + m); }
   template<int NR, int RR> __device__ __forceinline__ T2 st(int p, int k, int stride) const { return (*this)(p * stride * k); }
+  __device__ __forceinline__ T2 pk(int p, int k) const { return (*this)(p * k); }
 };
 
 // Exact product tables for the stages whose twiddles are n-th roots with n = Ns*R small (Ns <= 16
@@ -165,16 +166,39 @@ __device__ __forceinline__ constexpr int taEntry(int p) { return (p % 17 == 16) 
 // last, Ns = F2, and the second forward stage, Ns = F0 dividing F2) read them from the product
 // table of the (F1*F2)-th roots after it (fillProd: F2 rows, stride F1 - 1; at most T = W/F0 =
 // F1*F2 slots are reserved): phase p of such a stage reads row p * (F1*F2) / (Ns*F1).
+//
+// The other twiddles are w_W^(p*k) for a phase p and k < 16 (the last forward stage, Ns*R = W,
+// and the last inverse stage, p = tid, k < F0). TA[p*k & 63] for 16 consecutive p hits as few
+// as a quarter of the bank slots for odd k; from W = 1024 up they are instead
+// P2[(p >> 4) * k] * P1[p & 15][k] with P1[t][k] = w_W^(t*k) (t < 16, k < 16: a product table
+// as above, stride 15) and P2[j] = w_W^(16 j) (j < W/16; one entry for the 16 lanes of an access
+// phase): again the product of two correctly rounded entries, now without bank conflicts. Layout
+// from W = 1024: the (F1*F2)-th-root product table (T slots), P1 (240), P2 (W/16), no TA/TB.
 constexpr int ROWTW_SIZE = TA_SLOTS + 64;
-template<int T> constexpr int rowTwSlots() { return ROWTW_SIZE + T; }
-template<int F1, int F2>
+template<int W, int T> constexpr int rowTwSlots() { return W >= 1024 ? T + 240 + W / 16 : ROWTW_SIZE + T; }
+template<int W, int F1, int F2>
 struct RowTw {
   static_assert(F1 % 2 == 0 && F2 * (F1 - 1) <= F1 * F2, "row product table");
+  static constexpr bool PK = W >= 1024;
+  static constexpr int PT = PK ? 0 : ROWTW_SIZE, P1 = F1 * F2, P2 = F1 * F2 + 240;
   const T2* tab;
-  __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
+  __device__ __forceinline__ T2 operator()(int m) const {
+    static_assert(!PK, "no TA/TB table from W = 1024 up");
+    return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]);
+  }
+  // w_W^(p*k) for p*k < W and 1 <= k < 16
+  __device__ __forceinline__ T2 pk(int p, int k) const {
+    if constexpr (PK) {
+      return cmul(tab[P2 + (p >> 4) * k], tab[P1 + (p & 15) * 15 + k - 1]);
+    } else {
+      return (*this)(p * k);
+    }
+  }
   template<int NR, int RR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
     if constexpr (RR == F1 && (F1 * F2) % NR == 0) {
-      return tab[ROWTW_SIZE + p * ((F1 * F2) / NR) * (F1 - 1) + k - 1];
+      return tab[PT + p * ((F1 * F2) / NR) * (F1 - 1) + k - 1];
+    } else if constexpr (NR == W && RR <= 16) {
+      return pk(p, k);
     } else {
       return (*this)(p * stride * k);
     }
@@ -183,13 +207,19 @@ struct RowTw {
 template<int W, int T, int F1, int F2>
 __device__ __forceinline__ void fillRowTw(const T2* twRow, T2* tab, int tid) {
   static_assert(W % 64 == 0 && W / 64 <= 64 && W % T == 0, "twiddle table");
-  constexpr int N = TA_SLOTS + W / 64;
+  using TW = RowTw<W, F1, F2>;
+  if constexpr (TW::PK) {
+    fillProd<16, 15, 15>(twRow, 1, tab + TW::P1, tid, T);
+    for (int j = tid; j < W / 16; j += T) { tab[TW::P2 + j] = gld(twRow + 16 * j); }
+  } else {
+    constexpr int N = TA_SLOTS + W / 64;
 #pragma unroll
-  for (int p = tid; p < N; p += T) {
-    const int e = p < TA_SLOTS ? taEntry(p) : 64 * (p - TA_SLOTS);
-    tab[p] = e < 0 ? make_double2(0, 0) : gld(twRow + e);
+    for (int p = tid; p < N; p += T) {
+      const int e = p < TA_SLOTS ? taEntry(p) : 64 * (p - TA_SLOTS);
+      tab[p] = e < 0 ? make_double2(0, 0) : gld(twRow + e);
+    }
   }
-  fillProd<F2, F1 - 1, F1 - 1>(twRow, W / (F1 * F2), tab + ROWTW_SIZE, tid, T);
+  fillProd<F2, F1 - 1, F1 - 1>(twRow, W / (F1 * F2), tab + TW::PT, tid, T);
 }
 
 // Column-pass twiddles staged in LDS after the data regions (colTwSize<H>() entries):
@@ -205,8 +235,14 @@ template<int H> constexpr int colTwSize() { return TA_SLOTS + H / 64 + 2 * colTw
 // roots) and the second inverse stage (Ns = 16, R = H/256: (H/16)-th roots) read exact product
 // tables (fillProd; roots, after the pair tables): A, 16 rows of stride 15, then B, 16 rows of stride
 // (H/256 - 1) | 1, unless H/16 = 256 and A serves both.
+// Below H = 4096 the stages whose twiddles are w_H^(p*k) (the last forward stage, Ns = 256, and the
+// last inverse stage, Ns = H/16) read them as P2[(p >> 4) * k] * P1[p & 15][k], as the row pass
+// does (P1: w_H^(t*k), 16 rows of stride 15; P2: w_H^(16 j), j < H/16), after B. (At H = 4096
+// they stay with TA and TB: the two-pair pass would not fit in LDS.)
 template<int H> constexpr int colProdStrideB() { return (H / 256 - 1) | 1; }
-template<int H> constexpr int colRootSlots() { return 16 * 15 + (H == 4096 ? 0 : 16 * colProdStrideB<H>()); }
+template<int H> constexpr int colRootSlots() {
+  return 16 * 15 + (H == 4096 ? 0 : 16 * colProdStrideB<H>() + 240 + H / 16);
+}
 template<int H, bool ROOTS = false>
 struct ColTw {
   static_assert(H % 64 == 0 && H / 64 <= 64, "twiddle table");
@@ -214,11 +250,23 @@ struct ColTw {
   const T2* tab;
   const T2* roots = nullptr;
   __device__ __forceinline__ T2 operator()(int m) const { return cmul(tab[taPos(m & 63)], tab[TA_SLOTS + (m >> 6)]); }
+  static constexpr bool PK = ROOTS && H != 4096;
+  static constexpr int P1 = PK ? 16 * 15 + 16 * colProdStrideB<H>() : 0, P2 = P1 + 240;
+  // w_H^(p*k) for p*k < H and 1 <= k < 16
+  __device__ __forceinline__ T2 pk(int p, int k) const {
+    if constexpr (PK) {
+      return cmul(roots[P2 + (p >> 4) * k], roots[P1 + (p & 15) * 15 + k - 1]);
+    } else {
+      return (*this)(p * k);
+    }
+  }
   template<int NR, int RR> __device__ __forceinline__ T2 st(int p, int k, int stride) const {
     if constexpr (ROOTS && NR == 256 && RR == 16) {
       return roots[p * 15 + k - 1];
     } else if constexpr (ROOTS && NR == H / 16 && RR == H / 256) {
       return roots[16 * 15 + p * colProdStrideB<H>() + k - 1];
+    } else if constexpr (NR == H && RR <= 16) {
+      return pk(p, k);
     } else {
       return (*this)(p * stride * k);
     }
@@ -260,7 +308,12 @@ template<int H>
 __device__ __forceinline__ void fillColRoots(const JobDesc& J, T2* roots, int tid, int T) {
   static_assert(H % 256 == 0 && H / 256 <= 16, "column product tables");
   fillProd<16, 15, 15>(J.twCol, H / 256, roots, tid, T);
-  if constexpr (H != 4096) { fillProd<16, H / 256 - 1, colProdStrideB<H>()>(J.twCol, 16, roots + 16 * 15, tid, T); }
+  if constexpr (H != 4096) {
+    fillProd<16, H / 256 - 1, colProdStrideB<H>()>(J.twCol, 16, roots + 16 * 15, tid, T);
+    using TW = ColTw<H, true>;
+    fillProd<16, 15, 15>(J.twCol, 1, roots + TW::P1, tid, T);
+    for (int j = tid; j < H / 16; j += T) { roots[TW::P2 + j] = gld(J.twCol + 16 * j); }
+  }
 }
 
 // ---------------------------------------------------------------- column pass
@@ -444,7 +497,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
 #pragma unroll
           for (int k = 0; k < C0; ++k) { v[k] = lds[h * REG + padi(j + k * NB0)]; }
 #pragma unroll
-          for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw(j * k)); }
+          for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw.pk(j, k)); }
           Dft<C0, true>::run(v);
           const T2 base = tw.mid(h, j);
           const uint32_t o = offset0(q, h, j);
@@ -654,7 +707,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   static constexpr bool LDS_TW = W >= 512;
   // the block streams through with non-temporal accesses from W = 512 up
   static constexpr bool NT = W >= 512;
-  using Tw = std::conditional_t<LDS_TW, RowTw<F1, F2>, GlobalTw>;
+  using Tw = std::conditional_t<LDS_TW, RowTw<W, F1, F2>, GlobalTw>;
 
   // element e = s*R + r of the block (slot s, line r) for e = tid + i*T
   __device__ __forceinline__ static void load(const T2* blk, T2* P, int tid) {
@@ -705,7 +758,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
 #pragma unroll
       for (int k = 0; k < F0; ++k) { v[r][k] = lds[r * REG + padi(tid + k * T)]; }
 #pragma unroll
-      for (int k = 1; k < F0; ++k) { v[r][k] = cmulc(v[r][k], tw(tid * k)); }
+      for (int k = 1; k < F0; ++k) { v[r][k] = cmulc(v[r][k], tw.pk(tid, k)); }
       Dft<F0, true>::run(v[r]);
     }
   }
@@ -839,7 +892,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
 };
 
 // Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 + 64
-// (+ rowTwSlots<T>()*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle tables.
+// (+ rowTwSlots<W, T>()*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle tables.
 // Modes, options and the tagged stairway as k_row.
 template<int W, int R, int F0, int F1, int F2>
 __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ jobs, int mode, int fi, int wi, int opts,
