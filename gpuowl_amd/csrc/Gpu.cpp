@@ -141,9 +141,10 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(rowKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &rowT), "kernel");
-    // R regions of rowRegion<W>() (kfixed.h) + per-wave round-off + the rowTwSlots<W, rowT>()-entry
+    // R regions of rowRegion<W, R>() (kfixed.h) + per-wave round-off + the rowTwSlots<W, rowT>()-entry
     // twiddle tables (W >= 512)
-    rowLds = R * (W + W / 16 + 4) * 16 + 64 + (W >= 1024 ? (rowT + 240 + W / 16) * 16 : W >= 512 ? (132 + rowT) * 16 : 0);
+    rowLds = R * (W + W / 16 + ((R == 4 || R == 8) ? 6 : 4)) * 16 + 64 +
+             (W >= 1024 ? (rowT + 240 + W / 16) * 16 : W >= 512 ? (132 + rowT) * 16 : 0);
   } else {
     rowKernel = "row" + std::to_string(rowT);
   }
@@ -155,7 +156,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     const u32 twSize = 68 + H / 64 + 2 * (64 + H / 64 + 16);
     // the two-region pass adds exact root tables for H >= 2304 (colRootSlots<H>())
     const u32 rootSlots = (!useY && H >= 2304) ? 16 * 15 + (H == 4096 ? 0 : 16 * ((H / 256 - 1) | 1) + 240 + H / 16) : 0;
-    colLds = ((useY ? 1 : 2) * (H + H / 16 + 8) + twSize + rootSlots) * 16;
+    colLds = ((useY ? 1 : 2) * (H + H / 16 + 4) + twSize + rootSlots) * 16;
     if (useX && !opt.colOnePair && haveKernel(colX + "m")) {
       int cus = 0, lds = 0;
       owlCheck(owl_hip_device_info(opt.device, nullptr, 0, nullptr, nullptr, &cus, &lds), "device info");
@@ -223,6 +224,14 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     // Per row-pass thread and block: which of its words are big and which weight exponents wrap
     // (the tests of wordWeight, src/Gpu.cpp:40-127's bitsCF), so the pass does not recompute them.
     const u32 T = rowT, F0 = W / T;
+    // thread tid's columns are colBase(tid) + k*T (RowPass::colBase, kfixed.h)
+    auto colBase = [F0](u32 tid) -> u32 {
+      const u32 b = tid & 15, hi = tid & ~15u;
+      if (F0 == 2) { return hi | (b & 3) | ((b >> 3) << 2) | (((b >> 2) & 1) << 3); }
+      if (F0 == 4) { return hi | (b & 1) | (((b >> 3) & 1) << 1) | (((b >> 1) & 3) << 2); }
+      if (F0 == 8) { return hi | ((b >> 3) & 1) | ((b & 7) << 1); }
+      return tid;
+    };
     std::vector<u64> meta(size_t(nb) * T, 0);
     for (u32 lb = 0; lb < nb; ++lb) {
       for (u32 tid = 0; tid < T; ++tid) {
@@ -231,7 +240,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
           for (u32 r = 0; r < R; ++r) {
             for (u32 rep = 0; rep < 2; ++rep) {
               const u32 w = (k * R + r) * 2 + rep;
-              const u32 t = lineW[lb * R + r].e[rep] + colW[tid + k * T].e;
+              const u32 t = lineW[lb * R + r].e[rep] + colW[colBase(tid) + k * T].e;
               const bool wrap = t >= N;
               const bool big = (wrap ? t - N : t) < J.rE;
               m |= (u64(big) << w) | (u64(wrap) << (32 + w));

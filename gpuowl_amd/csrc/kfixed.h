@@ -12,11 +12,15 @@
 //           the carry chain, the stairway hand-off and the weighting all run in
 //           registers and feed the first forward stage directly
 //           (forward plan [F0,F1,F2], inverse plan [F2,F1,F0]).
-// LDS is padded by one element per 16 (index i -> i + i/16) so that the
-// stride-R writes of the first stages do not hit one bank, and consecutive
-// line (row pass) or column (column pass) regions start 4 or 8 elements apart
-// in the 64-bank cycle, so that the slot-order load and store phases, whose 16
-// lanes of a b128 access span several regions, spread over the banks.
+// LDS bank model (gfx950, 64 banks of 4 bytes, 16-byte accesses, fitted to rocprofv3's
+// SQ_LDS_IDX_ACTIVE over probe patterns): a read serves 32 lanes per two cycles, taking as many
+// cycles as the most distinct 16-byte slots among them that share a slot index mod 16; a write
+// serves 8 lanes per cycle, as many cycles as the most distinct slots sharing an index mod 8.
+// LDS is padded by one element per 16 (index i -> i + i/16) so that the strided accesses of the
+// FFT stages spread over the banks, and consecutive line (row pass) or column (column pass)
+// regions start 6 (row pass, 4 or 8 lines) or 4 elements apart mod 16, which keeps the writes
+// that span several regions (the row pass's slot-order scatter, the column pass's first stage)
+// on distinct slots mod 8.
 #pragma once
 
 #include <type_traits>
@@ -25,8 +29,8 @@
 
 __device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
 // region strides in 16-byte elements (the host sizes LDS with the same formulas)
-template<int W> constexpr int rowRegion() { return W + W / 16 + 4; }
-template<int H> constexpr int colRegion() { return H + H / 16 + 8; }
+template<int W, int R> constexpr int rowRegion() { return W + W / 16 + ((R == 4 || R == 8) ? 6 : 4); }
+template<int H> constexpr int colRegion() { return H + H / 16 + 4; }
 
 // One LDS -> LDS radix-R autosort stage over NFFT FFTs of length L stored in
 // padded regions; Ns is the product of the earlier radices. The twiddle of butterfly input k
@@ -703,11 +707,29 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   static constexpr int T = W / F0;
   static constexpr int LW = R * W;
   static constexpr int NL = LW / T;  // state elements per thread
-  static constexpr int REG = rowRegion<W>();
+  static constexpr int REG = rowRegion<W, R>();
   static constexpr bool LDS_TW = W >= 512;
   // the block streams through with non-temporal accesses from W = 512 up
   static constexpr bool NT = W >= 512;
   using Tw = std::conditional_t<LDS_TW, RowTw<W, F1, F2>, GlobalTw>;
+
+  // Thread tid owns the columns colBase(tid) + k*T (k < F0): colBase permutes the low four bits of
+  // tid so that the first forward stage's LDS writes (positions colBase*F0 + k, one per lane) of
+  // 8 consecutive lanes fall in 8 distinct bank slots (LDS writes serve 8 lanes per cycle, 16-byte
+  // slots mod 8); with colBase = tid they pair up for F0 = 2, 4 and 8. The host lays out the word
+  // metadata with the same permutation.
+  __device__ __forceinline__ static int colBase(int tid) {
+    const int b = tid & 15, hi = tid & ~15;
+    if constexpr (F0 == 2) {
+      return hi | (b & 3) | ((b >> 3) << 2) | (((b >> 2) & 1) << 3);
+    } else if constexpr (F0 == 4) {
+      return hi | (b & 1) | (((b >> 3) & 1) << 1) | (((b >> 1) & 3) << 2);
+    } else if constexpr (F0 == 8) {
+      return hi | ((b >> 3) & 1) | ((b & 7) << 1);
+    } else {
+      return tid;
+    }
+  }
 
   // element e = s*R + r of the block (slot s, line r) for e = tid + i*T
   __device__ __forceinline__ static void load(const T2* blk, T2* P, int tid) {
@@ -749,16 +771,16 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
     for (int i = 0; i < NL; ++i) { lds[pos(tid, i)] = P[i]; }
   }
   // inverse W-point FFT of the R lines (LDS stages, then the radix-F0 stage into registers)
-  __device__ __forceinline__ static void inverse(T2* lds, const Tw& tw, int tid, T2 (&v)[R][F0]) {
+  __device__ __forceinline__ static void inverse(T2* lds, const Tw& tw, int tid, int cb, T2 (&v)[R][F0]) {
     XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, tw, tid);
     __syncthreads();
-    // last inverse stage (radix F0, Ns = T): butterfly tid of each line -> columns tid + k*T
+    // last inverse stage (radix F0, Ns = T): butterfly cb of each line -> columns cb + k*T
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
-      for (int k = 0; k < F0; ++k) { v[r][k] = lds[r * REG + padi(tid + k * T)]; }
+      for (int k = 0; k < F0; ++k) { v[r][k] = lds[r * REG + padi(cb + k * T)]; }
 #pragma unroll
-      for (int k = 1; k < F0; ++k) { v[r][k] = cmulc(v[r][k], tw.pk(tid, k)); }
+      for (int k = 1; k < F0; ++k) { v[r][k] = cmulc(v[r][k], tw.pk(cb, k)); }
       Dft<F0, true>::run(v[r]);
     }
   }
@@ -851,11 +873,11 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   }
   // The other forward stages in LDS. The caller has made sure that no thread still reads the
   // line regions.
-  __device__ __forceinline__ static void forwardLds(const T2 (&v)[R][F0], T2* lds, const Tw& tw, int tid) {
+  __device__ __forceinline__ static void forwardLds(const T2 (&v)[R][F0], T2* lds, const Tw& tw, int tid, int cb) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
-      for (int k = 0; k < F0; ++k) { lds[r * REG + padi(tid * F0 + k)] = v[r][k]; }
+      for (int k = 0; k < F0; ++k) { lds[r * REG + padi(cb * F0 + k)] = v[r][k]; }
     }
     __syncthreads();
     XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, tw, tid);
@@ -869,29 +891,29 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   }
   // the stairway carries this block (g) waits for: block g-1's, shifted by one column for the
   // redo of block 0 (g == nb, the wrap-around)
-  __device__ __forceinline__ static void pollPredecessor(const JobDesc& J, int g, int tid, uint32_t tag, uint64_t* got) {
+  __device__ __forceinline__ static void pollPredecessor(const JobDesc& J, int g, int cb, uint32_t tag, uint64_t* got) {
     const int nb = J.nb;
     uint64_t* src[F0];
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
-      const int c = tid + k * T;
+      const int c = cb + k * T;
       src[k] = J.stair + (size_t)(g - 1) * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
     }
     pollAll<F0>(J, src, tag, got);
     if (g == nb) {  // block 0's first visit has read block 0 before the redo rewrites it
 #pragma unroll
-      for (int k = 0; k < F0; ++k) { src[k] = J.stair + tid + k * T; }
+      for (int k = 0; k < F0; ++k) { src[k] = J.stair + cb + k * T; }
       uint64_t seen[F0];
       pollAll<F0>(J, src, tag, seen);
     }
   }
-  __device__ __forceinline__ static void publish(const JobDesc& J, int g, int tid, uint32_t tag, const double* cs) {
+  __device__ __forceinline__ static void publish(const JobDesc& J, int g, int cb, uint32_t tag, const double* cs) {
 #pragma unroll
-    for (int k = 0; k < F0; ++k) { stairStore(J.stair + (size_t)g * W + tid + k * T, packCarry((int64_t)cs[k], tag)); }
+    for (int k = 0; k < F0; ++k) { stairStore(J.stair + (size_t)g * W + cb + k * T, packCarry((int64_t)cs[k], tag)); }
   }
 };
 
-// Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W>()*16 + 64
+// Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W, R>()*16 + 64
 // (+ rowTwSlots<W, T>()*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle tables.
 // Modes, options and the tagged stairway as k_row.
 template<int W, int R, int F0, int F1, int F2>
@@ -910,7 +932,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   // the block index is uniform: line weights then come through the scalar cache
   const int g = __builtin_amdgcn_readfirstlane((mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x);
   const int lb = (g == nb) ? 0 : g;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, cb = P::colBase(tid);
   const bool ll = opts & 1, mul3 = opts & 2;
   T2* blk = J.freq[fi] + (size_t)lb * LW;
   if constexpr (P::LDS_TW) { fillRowTw<W, T, F1, F2>(J.twRow, twTab, tid); }  // read after the first barrier below
@@ -918,7 +940,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   T2 v[R][F0];
   ColW cw[F0];
 #pragma unroll
-  for (int k = 0; k < F0; ++k) { cw[k] = gld(J.colW + tid + k * T); }
+  for (int k = 0; k < F0; ++k) { cw[k] = gld(J.colW + cb + k * T); }
   const uint64_t meta = gld(J.rowMeta + (size_t)lb * T + tid);
   const LineW* lwBase = J.lineW + lb * R;
 
@@ -928,7 +950,7 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
     for (int k = 0; k < F0; ++k) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int2 d = gld(src + (tid + k * T) * R + r);
+        const int2 d = gld(src + (cb + k * T) * R + r);
         v[r][k] = make_double2(d.x, d.y);
       }
     }
@@ -939,36 +961,36 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       P::scatter(lds, x, tid);
     }
     __syncthreads();
-    P::inverse(lds, tw, tid, v);
+    P::inverse(lds, tw, tid, cb, v);
     double roe = 0;
     double cs[F0];
-    P::carry(J, lwBase, cw, meta, v, cs, roe, ll && lb == 0 && tid == 0, mul3);
+    P::carry(J, lwBase, cw, meta, v, cs, roe, ll && lb == 0 && cb == 0, mul3);
     if (mode == ROW_CARRYOUT) {
       roeReport(J, roeSlot, (float)roe);
       int2* dst = J.words[wi] + (size_t)lb * LW;
 #pragma unroll
       for (int k = 0; k < F0; ++k) {
-        const int c = tid + k * T;
+        const int c = cb + k * T;
         gst(J.carry[0] + (size_t)lb * W + c, (int64_t)cs[k]);
 #pragma unroll
         for (int r = 0; r < R; ++r) { gst(dst + c * R + r, make_int2((int)v[r][k].x, (int)v[r][k].y)); }
       }
       return;
     }
-    if (g < nb) { P::publish(J, g, tid, tag, cs); }
+    if (g < nb) { P::publish(J, g, cb, tag, cs); }
     if (g == 0) {
       roeReport(J, roeSlot, (float)roe);
       return;
     }
     roeWave(roeSlot, (float)roe, roePart);
     uint64_t got[F0];
-    P::pollPredecessor(J, g, tid, tag, got);
+    P::pollPredecessor(J, g, cb, tag, got);
     P::ripple(J, meta, v, got);
   }
   P::weightStage0(lwBase, cw, meta, v);
   __syncthreads();  // every thread has finished reading the inverse's LDS
   if (mode == ROW_FUSED) { roeGroup(J, roeSlot, roePart, T / 64); }
-  P::forwardLds(v, lds, tw, tid);
+  P::forwardLds(v, lds, tw, tid, cb);
   P::store(blk, lds, tid);
 }
 
