@@ -152,11 +152,12 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
     owl_kernel k = nullptr;
     owlCheck(owl_hip_kernel_get(colKernel.c_str(), &k), "kernel");
     owlCheck(owl_hip_kernel_max_threads(k, &colT), "kernel");
-    // colRegion<H>() regions + a colTwSize<H>()-entry twiddle table (kfixed.h)
+    // the column regions + a colTwSize<H>()-entry twiddle table (kfixed.h)
     const u32 twSize = 68 + H / 64 + 2 * (64 + H / 64 + 16);
     // the two-region pass adds exact root tables for H >= 2304 (colRootSlots<H>())
     const u32 rootSlots = (!useY && H >= 2304) ? 16 * 15 + (H == 4096 ? 0 : 16 * ((H / 256 - 1) | 1) + 240 + H / 16) : 0;
-    colLds = ((useY ? 1 : 2) * (H + H / 16 + 4) + twSize + rootSlots) * 16;
+    // two colRegionX<H>() regions (k_colx) or one colRegion<H>() (k_coly)
+    colLds = ((useY ? H + H / 16 + 4 : 2 * (H + 4)) + twSize + rootSlots) * 16;
     if (useX && !opt.colOnePair && haveKernel(colX + "m")) {
       int cus = 0, lds = 0;
       owlCheck(owl_hip_device_info(opt.device, nullptr, 0, nullptr, nullptr, &cus, &lds), "device info");

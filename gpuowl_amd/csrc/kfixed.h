@@ -28,9 +28,24 @@
 #include "kcommon.h"
 
 __device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
+// Column-pass region layout: element i at lx(i), the low three bits of i XORed with bits 4..6
+// (each 16-element run keeps its place, permuted), no padding. Consecutive elements read by a
+// half-wave stay in distinct bank quads within each read sub-group, and the R consecutive outputs
+// of a radix-R butterfly for 8 consecutive lanes (stride R across the lanes) land on 8 distinct
+// slots mod 8. (In the row pass, whose LDS phases overlap the other group's work, the padded layout
+// measured as fast: there the fewer address operations weigh as much as the fewer bank conflicts.)
+__device__ __forceinline__ int lx(int i) { return i ^ ((i >> 4) & 7); }
+template<bool XL> __device__ __forceinline__ int lpos(int i) {
+  if constexpr (XL) {
+    return lx(i);
+  } else {
+    return padi(i);
+  }
+}
 // region strides in 16-byte elements (the host sizes LDS with the same formulas)
 template<int W, int R> constexpr int rowRegion() { return W + W / 16 + ((R == 4 || R == 8) ? 6 : 4); }
-template<int H> constexpr int colRegion() { return H + H / 16 + 4; }
+template<int H> constexpr int colRegion() { return H + H / 16 + 4; }  // k_coly (padded layout)
+template<int H> constexpr int colRegionX() { return H + 4; }          // k_colx (lx layout)
 
 // One LDS -> LDS radix-R autosort stage over NFFT FFTs of length L stored in
 // padded regions; Ns is the product of the earlier radices. The twiddle of butterfly input k
@@ -43,7 +58,7 @@ template<int H> constexpr int colRegion() { return H + H / 16 + 4; }
 // reads across waves.
 template<int L, int T, int R, bool PRIV> constexpr bool stagePrivate() { return PRIV && (T == 64 || L / R == 64); }
 
-template<int L, int REG, int NFFT, int T, bool INV, int R, int Ns, bool PRIV = false, class TW>
+template<int L, int REG, int NFFT, int T, bool INV, int R, int Ns, bool PRIV = false, bool XL = false, class TW>
 __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
   constexpr bool priv = stagePrivate<L, T, R, PRIV>();
   constexpr int NB = L / R;
@@ -57,7 +72,19 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
     if (TOT % T == 0 || g < TOT) {
       const int f = g / NB, j = g % NB;
       const T2* x = lds + f * REG;
-      if constexpr (NB % 16 == 0) {  // padi(j + k*NB) = padi(j) + k*(NB + NB/16)
+      if constexpr (XL && NB % 128 == 0) {  // lx(j + k*NB) = lx(j) + k*NB
+        const T2* xj = x + lx(j);
+#pragma unroll
+        for (int k = 0; k < R; ++k) { v[m][k] = xj[k * NB]; }
+      } else if constexpr (XL && NB % 64 == 0) {  // lx(j + k*NB) = (lx(j) ^ 4*(k & 1)) + k*NB
+        const T2* x0 = x + lx(j);
+        const T2* x1 = x + (lx(j) ^ 4);
+#pragma unroll
+        for (int k = 0; k < R; ++k) { v[m][k] = (k & 1 ? x1 : x0)[k * NB]; }
+      } else if constexpr (XL) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) { v[m][k] = x[lx(j + k * NB)]; }
+      } else if constexpr (NB % 16 == 0) {  // padi(j + k*NB) = padi(j) + k*(NB + NB/16)
         const T2* xj = x + padi(j);
 #pragma unroll
         for (int k = 0; k < R; ++k) { v[m][k] = xj[k * (NB + NB / 16)]; }
@@ -89,7 +116,18 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
       T2* x = lds + f * REG;
       const int p = j % Ns;
       const int base = (j - p) * R + p;
-      if constexpr (Ns % 16 == 0) {  // padi(base + k*Ns) = padi(base) + k*(Ns + Ns/16)
+      if constexpr (XL && Ns % 128 == 0) {  // lx(base + k*Ns) = lx(base) + k*Ns
+        T2* xb = x + lx(base);
+#pragma unroll
+        for (int k = 0; k < R; ++k) { xb[k * Ns] = v[m][k]; }
+      } else if constexpr (XL && Ns == 1 && 16 % R == 0) {  // base = j*R: lx(base + k) = lx(base) ^ k
+        const int lb = lx(base);
+#pragma unroll
+        for (int k = 0; k < R; ++k) { x[lb ^ k] = v[m][k]; }
+      } else if constexpr (XL) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) { x[lx(base + k * Ns)] = v[m][k]; }
+      } else if constexpr (Ns % 16 == 0) {  // padi(base + k*Ns) = padi(base) + k*(Ns + Ns/16)
         T2* xb = x + padi(base);
 #pragma unroll
         for (int k = 0; k < R; ++k) { xb[k * (Ns + Ns / 16)] = v[m][k]; }
@@ -110,20 +148,22 @@ __device__ __forceinline__ void xstage(T2* lds, const TW& tw, int tid) {
   }
 }
 
-template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, int Ns, int... Rs> struct XStages;
-template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, int Ns> struct XStages<L, REG, NFFT, T, INV, PRIV, Ns> {
+// XL: regions in the lx layout (row pass) instead of padi (column pass)
+template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, bool XL, int Ns, int... Rs> struct XStages;
+template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, bool XL, int Ns>
+struct XStages<L, REG, NFFT, T, INV, PRIV, XL, Ns> {
   template<class TW> __device__ __forceinline__ static void run(T2*, const TW&, int) {}
 };
-template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, int Ns, int R, int... Rs>
-struct XStages<L, REG, NFFT, T, INV, PRIV, Ns, R, Rs...> {
+template<int L, int REG, int NFFT, int T, bool INV, bool PRIV, bool XL, int Ns, int R, int... Rs>
+struct XStages<L, REG, NFFT, T, INV, PRIV, XL, Ns, R, Rs...> {
   template<class TW> __device__ __forceinline__ static void run(T2* lds, const TW& tw, int tid) {
-    xstage<L, REG, NFFT, T, INV, R, Ns, PRIV>(lds, tw, tid);
+    xstage<L, REG, NFFT, T, INV, R, Ns, PRIV, XL>(lds, tw, tid);
     if constexpr (sizeof...(Rs) > 0) {
       // a workgroup-wide stage after a wave-private one reads what other waves wrote
       constexpr int next[] = {Rs...};
       if constexpr (stagePrivate<L, T, R, PRIV>() && !stagePrivate<L, T, next[0], PRIV>()) { __syncthreads(); }
     }
-    XStages<L, REG, NFFT, T, INV, PRIV, Ns * R, Rs...>::run(lds, tw, tid);
+    XStages<L, REG, NFFT, T, INV, PRIV, XL, Ns * R, Rs...>::run(lds, tw, tid);
   }
 };
 
@@ -322,7 +362,7 @@ __device__ __forceinline__ void fillColRoots(const JobDesc& J, T2* roots, int ti
 
 // ---------------------------------------------------------------- column pass
 // MULTI = false: grid (W/2, batch), group q handles the column pair q; LDS: two
-// colRegion<H>() regions and one twiddle table (colTwSize<H>() entries).
+// colRegionX<H>() regions (lx layout) and one twiddle table (colTwSize<H>() entries).
 // MULTI = true: grid (G, batch) with G <= W/2, group x handles the pairs x, x + G, x + 2G,
 // ... below W/2; LDS: the two regions and two tables. T threads; modes as k_col.
 // For H >= 2304 the root tables (colRootSlots<H>() entries) follow the twiddle table(s).
@@ -336,7 +376,7 @@ template<int H, int T, int C0, int C1, int C2, bool MULTI = false>
 __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, int mode, int fi, int oi) {
   static_assert(C0 * C1 * C2 == H && C0 == 16, "plan");
   extern __shared__ T2 lds[];
-  constexpr int REG = colRegion<H>();
+  constexpr int REG = colRegionX<H>();
   constexpr int NB0 = H / C0;
   constexpr int NBT = 2 * NB0;
   constexpr int MB0 = (NBT + T - 1) / T;
@@ -410,7 +450,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
         for (int k = 0; k < C0; ++k) { v[k] = cmul(xin[m][k], k ? cmul(base, tw.s(h, k)) : base); }
         Dft<C0, false>::run(v);
 #pragma unroll
-        for (int k = 0; k < C0; ++k) { lds[h * REG + padi(j * C0 + k)] = v[k]; }
+        for (int k = 0; k < C0; ++k) { lds[h * REG + (lx(j * C0) ^ k)] = v[k]; }
       }
     }
     // prefetch of the next pair: table entries first (written to LDS below, so their wait
@@ -425,7 +465,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
       loadPair(qn, tid);
     }
     __syncthreads();
-    XStages<H, REG, 2, T, false, false, C0, C1, C2>::run(lds, tw, tid);
+    XStages<H, REG, 2, T, false, false, true, C0, C1, C2>::run(lds, tw, tid);
     if (MULTI && qn < W / 2) {
       T2* tabn = lds + 2 * REG + ((it + 1) & 1) * TWS;
 #pragma unroll
@@ -441,54 +481,54 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
         int wi = e - lb * per;
         int h = wi >> lgR;
         int b = lb * R + (wi & (R - 1));
-        stx<NT>(io + (size_t)lb * blockStride + colBase + wi, lds[h * REG + padi(b)]);
+        stx<NT>(io + (size_t)lb * blockStride + colBase + wi, lds[h * REG + lx(b)]);
       }
     } else {
       auto spos = [&](int h, int b) -> size_t { return (size_t)(b >> lgR) * blockStride + colBase + (h << lgR) + (b & (R - 1)); };
       if (q) {
         for (int b = tid; b < H; b += T) {
           int bp = H - 1 - b;
-          T2 X = lds[padi(b)], Xp = lds[REG + padi(bp)];
+          T2 X = lds[lx(b)], Xp = lds[REG + lx(bp)];
           T2 w = cmul(wa, tw(b));
           if (other) {
             pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(1, bp)), w);
           } else {
             pairSq(X, Xp, w);
           }
-          lds[padi(b)] = X;
-          lds[REG + padi(bp)] = Xp;
+          lds[lx(b)] = X;
+          lds[REG + lx(bp)] = Xp;
         }
       } else {
         for (int i = tid; i <= H; i += T) {
           if (i <= H / 2) {
             int b = i, bp = (H - i) % H;
-            T2 X = lds[padi(b)], Xp = lds[padi(bp)];
+            T2 X = lds[lx(b)], Xp = lds[lx(bp)];
             T2 w = tw(b);
             if (other) {
               pairMul(X, Xp, gld(other + spos(0, b)), gld(other + spos(0, bp)), w);
             } else {
               pairSq(X, Xp, w);
             }
-            lds[padi(b)] = X;
-            if (bp != b) { lds[padi(bp)] = Xp; }
+            lds[lx(b)] = X;
+            if (bp != b) { lds[lx(bp)] = Xp; }
           } else {
             int b = i - H / 2 - 1, bp = H - 1 - b;
-            T2 X = lds[REG + padi(b)], Xp = lds[REG + padi(bp)];
+            T2 X = lds[REG + lx(b)], Xp = lds[REG + lx(bp)];
             T2 w = cmul(wa, tw(b));
             if (other) {
               pairMul(X, Xp, gld(other + spos(1, b)), gld(other + spos(1, bp)), w);
             } else {
               pairSq(X, Xp, w);
             }
-            lds[REG + padi(b)] = X;
-            lds[REG + padi(bp)] = Xp;
+            lds[REG + lx(b)] = X;
+            lds[REG + lx(bp)] = Xp;
           }
         }
       }
       __syncthreads();
       // inverse stages radix C1 then C2 (the reverse of the forward order would put the radix-C2
       // stage first, whose stride-C2 writes hit few LDS banks for C2 = 15)
-      XStages<H, REG, 2, T, true, false, 1, C1, C2>::run(lds, tw, tid);
+      XStages<H, REG, 2, T, true, false, true, 1, C1, C2>::run(lds, tw, tid);
 
       // last inverse stage (radix C0, Ns = NB0), fused with the conjugate mid twiddle and the store
 #pragma unroll
@@ -499,7 +539,7 @@ __global__ __launch_bounds__(T) void k_colx(const JobDesc* __restrict__ jobs, in
           const int j = ((g >> (lgR + 1)) << lgR) | (g & (R - 1));
           T2 v[C0];
 #pragma unroll
-          for (int k = 0; k < C0; ++k) { v[k] = lds[h * REG + padi(j + k * NB0)]; }
+          for (int k = 0; k < C0; ++k) { v[k] = lds[h * REG + lx(j + k * NB0)]; }
 #pragma unroll
           for (int k = 1; k < C0; ++k) { v[k] = cmulc(v[k], tw.pk(j, k)); }
           Dft<C0, true>::run(v);
@@ -566,7 +606,7 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
       for (int k = 0; k < C0; ++k) { lds[padi(j * C0 + k)] = v[k]; }
     }
     __syncthreads();
-    XStages<H, REG, 1, T, false, false, C0, C1>::run(lds, tw, tid);
+    XStages<H, REG, 1, T, false, false, false, C0, C1>::run(lds, tw, tid);
 #pragma unroll
     for (int k = 0; k < C2; ++k) { out[k] = lds[padi(jl + k * T)]; }
 #pragma unroll
@@ -581,7 +621,7 @@ __global__ __launch_bounds__(H / C2, 2) void k_coly(const JobDesc* __restrict__ 
 #pragma unroll
     for (int k = 0; k < C2; ++k) { lds[padi(jl * C2 + k)] = in[k]; }
     __syncthreads();
-    XStages<H, REG, 1, T, true, false, C2, C1>::run(lds, tw, tid);
+    XStages<H, REG, 1, T, true, false, false, C2, C1>::run(lds, tw, tid);
     if (tid < NB0) {
       const int j = tid;
       T2 v[C0];
@@ -772,7 +812,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   }
   // inverse W-point FFT of the R lines (LDS stages, then the radix-F0 stage into registers)
   __device__ __forceinline__ static void inverse(T2* lds, const Tw& tw, int tid, int cb, T2 (&v)[R][F0]) {
-    XStages<W, REG, R, T, true, true, 1, F2, F1>::run(lds, tw, tid);
+    XStages<W, REG, R, T, true, true, false, 1, F2, F1>::run(lds, tw, tid);
     __syncthreads();
     // last inverse stage (radix F0, Ns = T): butterfly cb of each line -> columns cb + k*T
 #pragma unroll
@@ -880,7 +920,7 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
       for (int k = 0; k < F0; ++k) { lds[r * REG + padi(cb * F0 + k)] = v[r][k]; }
     }
     __syncthreads();
-    XStages<W, REG, R, T, false, true, F0, F1, F2>::run(lds, tw, tid);
+    XStages<W, REG, R, T, false, true, false, F0, F1, F2>::run(lds, tw, tid);
     __syncthreads();
   }
   // the block from the line regions, slot order
