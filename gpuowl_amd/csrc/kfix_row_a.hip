@@ -3,8 +3,8 @@
 #include "kfixed.h"
 
 extern "C" const OwlKernelEntry owl_kernel_table_row_a[] = {
-    {"rowx256r2", (const void*)&k_rowx<256, 2, 4, 4, 16>, 64},
-    {"rowx512r2", (const void*)&k_rowx<512, 2, 4, 8, 16>, 128},
+    {"rowx256r2", (const void*)&k_rowx<256, 2, 2, 8, 16>, 128},
+    {"rowx512r2", (const void*)&k_rowx<512, 2, 2, 16, 16>, 256},
     {"rowx1024r2", (const void*)&k_rowx<1024, 2, 4, 16, 16>, 256},
     {"rowx2048r1", (const void*)&k_rowx<2048, 1, 8, 16, 16>, 256},
     {"rowx2048r2", (const void*)&k_rowx<2048, 2, 8, 16, 16>, 256},

@@ -3,7 +3,7 @@
 #include "kfixed.h"
 
 extern "C" const OwlKernelEntry owl_kernel_table_row_b[] = {
-    {"rowx512r4", (const void*)&k_rowx<512, 4, 4, 8, 16>, 128},
+    {"rowx512r4", (const void*)&k_rowx<512, 4, 2, 16, 16>, 256},
     {"rowx1024r4", (const void*)&k_rowx<1024, 4, 4, 16, 16>, 256},
     {nullptr, nullptr, 0},
 };

@@ -3,7 +3,7 @@
 #include "kfixed.h"
 
 extern "C" const OwlKernelEntry owl_kernel_table_row_c[] = {
-    {"rowx256r4", (const void*)&k_rowx<256, 4, 4, 4, 16>, 64},
+    {"rowx256r4", (const void*)&k_rowx<256, 4, 2, 8, 16>, 128},
     {"rowx256r8", (const void*)&k_rowx<256, 8, 2, 8, 16>, 128},
     {nullptr, nullptr, 0},
 };
