@@ -114,7 +114,6 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
 
   // shape-specialized passes when compiled for this shape (and R fits their lane mapping)
   rowKernel = "rowx" + std::to_string(W) + "r" + std::to_string(R);
-  if (opt.batched && haveKernel(rowKernel + "b")) { rowKernel += "b"; }
   bool rowFixed = !opt.generic && haveKernel(rowKernel);
   // column pass: k_colx (both columns in LDS) unless the one-region k_coly is asked for
   // or k_colx's lane mapping does not fit ((H/16) % R == 0); then k_coly if compiled

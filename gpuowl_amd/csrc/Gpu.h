@@ -48,8 +48,6 @@ struct GpuOptions {
   bool colOnePair = false;    // k_colx with one column pair per group where two would be chosen
   u32 width = 0;           // W of the W x H split of the FFT; 0 = the reference's (refactored when too tall)
   u32 lineBlock = 0;       // lines per row-pass block; 0 = automatic
-  bool batched = false;    // the job shares launches with others (batch, server): a row pass compiled
-                           // for many groups per launch ("rowx{W}r{R}b") where one exists
   int device = 0;
 };
 

@@ -23,7 +23,6 @@ GpuBatch::GpuBatch(const std::vector<u32>& exponents, const std::string& spec, c
   for (u32 E : exponents) {
     FFTConfig fft = FFTConfig::bestFit(E, spec);
     GpuOptions o = opt;
-    o.batched = true;
     TuneChoice c;
     // a split tuned for batches of this FFT length, when the caller chose nothing
     if (!o.width && !o.lineBlock && !o.generic && !o.colOneRegion && !o.colOnePair &&

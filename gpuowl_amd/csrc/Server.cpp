@@ -92,9 +92,7 @@ void Server::admit() {
     job->res.tSubmit = r.tSubmit;
     job->res.tAdmit = now();
     try {
-      GpuOptions o = opt_.gpu;
-      o.batched = true;
-      job->gpu = std::make_unique<Gpu>(r.E, fft, o);
+      job->gpu = std::make_unique<Gpu>(r.E, fft, opt_.gpu);
     } catch (const std::invalid_argument&) {
       job->res.status = 2;  // rejected: no engine for this exponent
       job->res.tDone = now();

@@ -1,7 +1,5 @@
 // Row passes compiled per (W, R): k_rowx<W, R, F0, F1, F2>, T = W/F0 threads.
-// The host picks "rowx<W>r<R>" when it exists; jobs that share launches (batches, the server)
-// take "rowx<W>r<R>b" where one is compiled: fewer threads per group, measured faster when a
-// launch's groups already fill the device, slower for one job alone.
+// The host picks "rowx<W>r<R>" when it exists.
 #include "kfixed.h"
 
 extern "C" const OwlKernelEntry owl_kernel_table_row_a[] = {

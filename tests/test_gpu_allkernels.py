@@ -1,7 +1,6 @@
 """Every compiled shape-specialized pass, forced through the tune table onto a geometry
 it serves, gives GMP's PRP residues (with Gerbicz checks) over a short run; the multi-pair
-column passes run in a batch large enough to choose them, the batch row passes ("...b") in a
-two-job batch."""
+column passes run in a batch large enough to choose them."""
 import re
 import sys
 
@@ -53,23 +52,6 @@ def test_specialized_kernel_is_exact(name):
     cfg = config_for(name)
     assert cfg and cfg[0], f"no reference shape serves {name}"
     spec, W, R, flags, row, col = cfg
-    if row and name.endswith("b"):
-        # the row pass for jobs that share launches: a two-job batch takes it
-        N = owl.best_fit(1000, spec)[1]
-        e0 = workloads.next_prime(15 * N)
-        exps = [e0, workloads.next_prime(e0 + 2)]
-        announce(name, spec, W, R, exps, "batch")
-        owl.tune_set(N, W, R, flags)
-        try:
-            with owl.Batch(exps, spec) as b:
-                assert [b.job_info(i)["row"] for i in range(2)] == [name, name]
-                outs, _ = b.prp(60, block_size=30, check_step=30)
-            for E, o in zip(exps, outs):
-                assert o.res64 == oracle.gmp_res64(E, 60, 60)[0], E
-                assert o.nErrors == 0 and o.maxRoe < 0.4
-        finally:
-            owl.tune_clear()
-        return
     if name.endswith("m"):
         # the multi-pair column pass (compiled for H >= 2304, one group per CU) runs where a
         # launch has groups to spare: one job already gives W/4 = 256 groups of two pairs
