@@ -115,6 +115,7 @@ Gpu::Gpu(u32 E, const FFTConfig& fftIn, const GpuOptions& optIn) : E{E}, fft{fft
   // shape-specialized passes when compiled for this shape (and R fits their lane mapping)
   rowKernel = "rowx" + std::to_string(W) + "r" + std::to_string(R);
   bool rowFixed = !opt.generic && haveKernel(rowKernel);
+  rowChunked = rowFixed && nb >= 8;
   // column pass: k_colx (both columns in LDS) unless the one-region k_coly is asked for
   // or k_colx's lane mapping does not fit ((H/16) % R == 0); then k_coly if compiled
   const std::string colY = "coly" + std::to_string(H), colX = "colx" + std::to_string(H);
@@ -380,9 +381,12 @@ std::string Gpu::shapeKey() const {
 }
 
 void Gpu::rowFused(bool ll, u32 roeSlot, Launch L) {
-  int mode = ROW_FUSED, fi = 0, wi = 0, opts = ll ? 1 : 0;
+  // k_rowx: eight chunks of blocks, one per XCD, ceil(nb/8) + 1 groups each (kfixed.h, fusedItem);
+  // the runtime-plan k_row: nb + 1 groups in start-ticket order
+  int mode = ROW_FUSED, fi = 0, wi = 0, opts = (ll ? 1 : 0) | (rowChunked ? 4 : 0);
   u32 tag = nextStairTag();
-  kRow(L.q, nb + 1, L.n, rowT, rowLds, {&L.jobs, &mode, &fi, &wi, &opts, &roeSlot, &tag});
+  kRow(L.q, rowChunked ? 8 * ((nb + 7) / 8 + 1) : nb + 1, L.n, rowT, rowLds,
+       {&L.jobs, &mode, &fi, &wi, &opts, &roeSlot, &tag});
 }
 
 void Gpu::rowLeadInTo(Buf in, int freqIdx, Launch L) {

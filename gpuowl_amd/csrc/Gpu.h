@@ -83,6 +83,7 @@ public:
   int device = 0;  // the HIP device the engine's queue and buffers are on
   u32 N, ND, W, H, R, nb, rowT, colT, rowLds, colLds;
   std::string rowKernel, colKernel;  // colKernel: what a launch of this job alone runs
+  bool rowChunked = false;           // fused row launches in chunked per-XCD block order (k_rowx, nb >= 8)
   // Column pass with several column pairs per group ("colx<H>m", compiled for every k_colx
   // shape): colLdsM bytes of LDS (one more twiddle table), slotsM = groups the device holds
   // at once. colPairsPerGroup(n) is 1 (the one-pair "colx<H>") unless a launch of n jobs

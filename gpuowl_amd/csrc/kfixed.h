@@ -740,6 +740,55 @@ __device__ __forceinline__ void pollAll(const JobDesc& J, uint64_t* const* p, ui
   }
 }
 
+// What one group of a fused row launch does: block lb; for a first visit (first) only the
+// carry-out is published, for a redo (redo) the block is transformed again and nothing is
+// published; otherwise it waits for block pred's carries (wrap: the carries of the last block,
+// which enter block 0 one column to the right) and publishes its own.
+struct FusedItem {
+  int lb, pred;
+  bool first, redo, wrap, none;
+};
+// Chunked order (opts bit 4, nb >= 8, grid 8 * (ceil(nb/8) + 1)): group i runs on XCD i mod 8
+// (workgroups go to the eight XCDs in turn) and XCD x works through one contiguous chunk of
+// blocks B_x .. B_{x+1}-1: its first group visits B_x first (publish only), the next groups take
+// the following blocks, and its last group redoes B_{x+1}, the head of the next chunk (block 0
+// after the last chunk) once the chunk's last carries are in. Every wait is then on a group of
+// the same XCD with a lower index, which that XCD has already started, or on the first group of
+// a chunk, which never waits: no start counter is needed. Groups beyond a chunk's items exit.
+// Ticket order (otherwise, grid nb + 1): the group's start ticket is its block; ticket nb redoes
+// block 0.
+__device__ __forceinline__ FusedItem fusedItem(const JobDesc& J, int opts, int* slot) {
+  const int nb = J.nb;
+  FusedItem it{0, 0, false, false, false, false};
+  if (opts & 4) {
+    const int x = blockIdx.x & 7, j = blockIdx.x >> 3, q = nb >> 3, rem = nb & 7;
+    const int b0 = x * q + min(x, rem), c = q + (x < rem ? 1 : 0);
+    if (j > c) {
+      it.none = true;
+    } else if (j == 0) {
+      it.lb = b0;
+      it.first = true;
+    } else if (j < c) {
+      it.lb = b0 + j;
+      it.pred = it.lb - 1;
+    } else {
+      const int h = b0 + c;  // the next chunk's head; nb after the last chunk
+      it.lb = (h == nb) ? 0 : h;
+      it.pred = h - 1;
+      it.redo = true;
+      it.wrap = h == nb;
+    }
+  } else {
+    const int g = __builtin_amdgcn_readfirstlane(fusedBlock(J, slot));
+    it.lb = (g == nb) ? 0 : g;
+    it.pred = g - 1;
+    it.first = g == 0;
+    it.redo = g == nb;
+    it.wrap = g == nb;
+  }
+  return it;
+}
+
 // Pieces of the row pass over one block of R lines, T = W/F0 threads. Thread tid owns columns
 // c = tid + k*T (k < F0) of the R lines; v[r][k] holds the inverse transform of (line r,
 // column c), then its balanced digits (as doubles), then the weighted digits.
@@ -929,20 +978,20 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
 #pragma unroll
     for (int i = 0; i < NL; ++i) { stx<NT>(blk + tid + i * T, lds[pos(tid, i)]); }
   }
-  // the stairway carries this block (g) waits for: block g-1's, shifted by one column for the
-  // redo of block 0 (g == nb, the wrap-around)
-  __device__ __forceinline__ static void pollPredecessor(const JobDesc& J, int g, int cb, uint32_t tag, uint64_t* got) {
-    const int nb = J.nb;
+  // the stairway carries this block waits for: block it.pred's, shifted by one column when they
+  // wrap around into block 0
+  __device__ __forceinline__ static void pollPredecessor(const JobDesc& J, const FusedItem& it, int cb, uint32_t tag,
+                                                         uint64_t* got) {
     uint64_t* src[F0];
 #pragma unroll
     for (int k = 0; k < F0; ++k) {
       const int c = cb + k * T;
-      src[k] = J.stair + (size_t)(g - 1) * W + ((g == nb) ? (c == 0 ? W - 1 : c - 1) : c);
+      src[k] = J.stair + (size_t)it.pred * W + (it.wrap ? (c == 0 ? W - 1 : c - 1) : c);
     }
     pollAll<F0>(J, src, tag, got);
-    if (g == nb) {  // block 0's first visit has read block 0 before the redo rewrites it
+    if (it.redo) {  // the block's first visit has read the block before the redo rewrites it
 #pragma unroll
-      for (int k = 0; k < F0; ++k) { src[k] = J.stair + cb + k * T; }
+      for (int k = 0; k < F0; ++k) { src[k] = J.stair + (size_t)it.lb * W + cb + k * T; }
       uint64_t seen[F0];
       pollAll<F0>(J, src, tag, seen);
     }
@@ -953,7 +1002,8 @@ template<int W, int R, int F0, int F1, int F2> struct RowPass {
   }
 };
 
-// Grid (nb + 1 for ROW_FUSED else nb, batch), T = W/F0 threads, LDS R*rowRegion<W, R>()*16 + 64
+// Grid (ROW_FUSED: nb + 1, or 8 * (ceil(nb/8) + 1) in chunked order; else nb; batch), T = W/F0
+// threads, LDS R*rowRegion<W, R>()*16 + 64
 // (+ rowTwSlots<W, T>()*16 for W >= 512) bytes: line regions, per-wave round-off, twiddle tables.
 // Modes, options and the tagged stairway as k_row.
 template<int W, int R, int F0, int F1, int F2>
@@ -970,8 +1020,10 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
   const int nb = J.nb;
   __shared__ int ticketSlot;
   // the block index is uniform: line weights then come through the scalar cache
-  const int g = __builtin_amdgcn_readfirstlane((mode == ROW_FUSED) ? fusedBlock(J, &ticketSlot) : (int)blockIdx.x);
-  const int lb = (g == nb) ? 0 : g;
+  FusedItem it{(int)blockIdx.x, 0, false, false, false, false};
+  if (mode == ROW_FUSED) { it = fusedItem(J, opts, &ticketSlot); }
+  if (it.none) { return; }
+  const int lb = it.lb;
   const int tid = threadIdx.x, cb = P::colBase(tid);
   const bool ll = opts & 1, mul3 = opts & 2;
   T2* blk = J.freq[fi] + (size_t)lb * LW;
@@ -1017,14 +1069,14 @@ __global__ __launch_bounds__(W / F0) void k_rowx(const JobDesc* __restrict__ job
       }
       return;
     }
-    if (g < nb) { P::publish(J, g, cb, tag, cs); }
-    if (g == 0) {
+    if (!it.redo) { P::publish(J, lb, cb, tag, cs); }
+    if (it.first) {
       roeReport(J, roeSlot, (float)roe);
       return;
     }
     roeWave(roeSlot, (float)roe, roePart);
     uint64_t got[F0];
-    P::pollPredecessor(J, g, cb, tag, got);
+    P::pollPredecessor(J, it, cb, tag, got);
     P::ripple(J, meta, v, got);
   }
   P::weightStage0(lwBase, cw, meta, v);
