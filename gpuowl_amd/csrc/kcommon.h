@@ -278,8 +278,9 @@ __device__ __forceinline__ uint64_t pollCarry(const JobDesc& J, uint64_t* p, uin
   return v;
 }
 
-// The fused row pass orders its groups by start, not by blockIdx: each group takes the
-// next ticket of the job's counter, and the ticket is its block. A group then only waits
+// Ticket order of the fused row pass (the runtime-plan k_row, and k_rowx below 8 blocks; k_rowx
+// otherwise uses the chunked per-XCD order of fusedItem): groups are ordered by start, not by
+// blockIdx: each group takes the next ticket of the job's counter, and the ticket is its block. A group then only waits
 // for groups that have already started, so the carry hand-off progresses even when other
 // kernels (other queues) hold the rest of the device. Every fused launch takes exactly
 // nb + 1 tickets; the group that takes the last one (no other group of the launch touches
