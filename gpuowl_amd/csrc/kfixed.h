@@ -757,35 +757,39 @@ struct FusedItem {
 // a chunk, which never waits: no start counter is needed. Groups beyond a chunk's items exit.
 // Ticket order (otherwise, grid nb + 1): the group's start ticket is its block; ticket nb redoes
 // block 0.
+// Item of group i of a chunked launch over nb >= 8 blocks (also compiled for the host, where a
+// CPU test checks the mapping).
+__host__ __device__ inline FusedItem chunkedItem(int nb, int i) {
+  FusedItem it{0, 0, false, false, false, false};
+  const int x = i & 7, j = i >> 3, q = nb >> 3, rem = nb & 7;
+  const int b0 = x * q + (x < rem ? x : rem), c = q + (x < rem ? 1 : 0);
+  if (j > c) {
+    it.none = true;
+  } else if (j == 0) {
+    it.lb = b0;
+    it.first = true;
+  } else if (j < c) {
+    it.lb = b0 + j;
+    it.pred = it.lb - 1;
+  } else {
+    const int h = b0 + c;  // the next chunk's head; nb after the last chunk
+    it.lb = (h == nb) ? 0 : h;
+    it.pred = h - 1;
+    it.redo = true;
+    it.wrap = h == nb;
+  }
+  return it;
+}
 __device__ __forceinline__ FusedItem fusedItem(const JobDesc& J, int opts, int* slot) {
   const int nb = J.nb;
+  if (opts & 4) { return chunkedItem(nb, blockIdx.x); }
   FusedItem it{0, 0, false, false, false, false};
-  if (opts & 4) {
-    const int x = blockIdx.x & 7, j = blockIdx.x >> 3, q = nb >> 3, rem = nb & 7;
-    const int b0 = x * q + min(x, rem), c = q + (x < rem ? 1 : 0);
-    if (j > c) {
-      it.none = true;
-    } else if (j == 0) {
-      it.lb = b0;
-      it.first = true;
-    } else if (j < c) {
-      it.lb = b0 + j;
-      it.pred = it.lb - 1;
-    } else {
-      const int h = b0 + c;  // the next chunk's head; nb after the last chunk
-      it.lb = (h == nb) ? 0 : h;
-      it.pred = h - 1;
-      it.redo = true;
-      it.wrap = h == nb;
-    }
-  } else {
-    const int g = __builtin_amdgcn_readfirstlane(fusedBlock(J, slot));
-    it.lb = (g == nb) ? 0 : g;
-    it.pred = g - 1;
-    it.first = g == 0;
-    it.redo = g == nb;
-    it.wrap = g == nb;
-  }
+  const int g = __builtin_amdgcn_readfirstlane(fusedBlock(J, slot));
+  it.lb = (g == nb) ? 0 : g;
+  it.pred = g - 1;
+  it.first = g == 0;
+  it.redo = g == nb;
+  it.wrap = g == nb;
   return it;
 }
 
