@@ -755,6 +755,8 @@ struct FusedItem {
 // after the last chunk) once the chunk's last carries are in. Every wait is then on a group of
 // the same XCD with a lower index, which that XCD has already started, or on the first group of
 // a chunk, which never waits: no start counter is needed. Groups beyond a chunk's items exit.
+// Results do not depend on the XCD assignment (any group order gives the same residues); only
+// forward progress while kernels of other queues hold parts of the device relies on it.
 // Ticket order (otherwise, grid nb + 1): the group's start ticket is its block; ticket nb redoes
 // block 0.
 // Item of group i of a chunked launch over nb >= 8 blocks (also compiled for the host, where a
